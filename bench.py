@@ -1,0 +1,211 @@
+"""Benchmark of the MI355X LULESH Lagrange leapfrog cycle.
+
+Workload: plain FP64 Sedov blast at s=150 (3,375,000 elements, 151^3 nodes,
+11 regions, -b 1 -c 1), the configuration the headline metric is quoted on.
+A step is one cycle (device TimeIncrement + LagrangeLeapFrog) over the whole
+mesh; the fields are resident in HBM before the timed region starts.
+
+Also reported on the same GPU: the FP32-mixed form at s=150 (with its
+origin-energy error against the FP64 path after the same cycles), per-kernel
+HIP-event timings and the dominant kernel's roofline, and the CPU oracle
+(OpenMP port of the reference arithmetic) timed on this host's cores.
+
+Prints one JSON line.
+"""
+import argparse
+import ctypes
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "lulesh-task-based-hpx_amd")
+HBM_PEAK_GBS = 8000.0
+
+
+def load_pylulesh():
+    spec = importlib.util.spec_from_file_location("pylulesh", os.path.join(PKG, "pylulesh.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["pylulesh"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def model_bytes_per_zone_cycle(s):
+    """Compulsory HBM bytes per zone-cycle of the minimal 4-pass schedule:
+    B(s) = 816 + 204 (s+1)^3 / s^3."""
+    return 816.0 + 204.0 * (s + 1) ** 3 / s ** 3
+
+
+def kernel_bytes(name, E, N, mixed=False):
+    """Algorithmic bytes moved by one launch of each kernel over the mesh
+    (every array element counted once per launch; node gathers once per
+    unique node)."""
+    hg = 4 if mixed else 8
+    gr = 4 if mixed else 8
+    if name == "force":
+        # nodelist, 6 node fields, p q volo v ss elemMass -> 24 stress + 24 hourglass corners
+        return E * (32 + 48 + 24 * 8 + 24 * hg) + N * 48
+    if name == "nodes":
+        # CSR start, corner map, 6 corner arrays, mass, bc mask, 6 node fields in,
+        # 12 node fields out (x..zd, fx..fz, xdd..zdd)
+        return E * (32 + 24 * 8 + 24 * hg) + N * (4 + 8 + 1 + 48 + 96)
+    if name == "elements":
+        return E * (32 + 16 + 40 + 6 * gr) + N * 48
+    if name == "material":
+        return E * (4 + 4 + 24 + 6 * gr + 72 + 48)
+    if name == "time_increment":
+        return 64
+    return 0
+
+
+def cpu_baseline(s, nr, seconds, threads):
+    """OpenMP oracle (the reference arithmetic) on the host: zone-cycles/s."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import OracleDomain, set_threads
+
+    set_threads(threads)
+    d = OracleDomain(s, nr)
+    d.run(1)  # cycle 1 outside the timing (first-touch of every array)
+    cycles, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        target = d.cycle + 2
+        d.run(target)
+        cycles += 2
+    el = time.perf_counter() - t0
+    d.close()
+    return s ** 3 * cycles / el, cycles, el
+
+
+def timed_run(pyl, dev, dom, warmup, steps):
+    ts = dom.time_state()
+    if warmup:
+        rc = dev.run(warmup, ts)
+        assert rc == 0, rc
+    t0 = time.perf_counter()
+    rc = dev.run(ts.cycle + steps, ts)
+    t1 = time.perf_counter()
+    assert rc == 0, rc
+    return t1 - t0, ts
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--s", type=int, default=150)
+    ap.add_argument("--regions", type=int, default=11)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
+              "decomposition is a design sketch, not built.", file=sys.stderr)
+        sys.exit(2)
+
+    pyl = load_pylulesh()
+    s, nr = args.s, args.regions
+    E = s ** 3
+    N = (s + 1) ** 3
+
+    # ---- FP64 headline -------------------------------------------------
+    print(f"[bench] building Domain s={s}", file=sys.stderr, flush=True)
+    dom = pyl.Domain(s, nr)
+    dev = pyl.Device(dom, pyl.FP64)
+    elapsed, ts = timed_run(pyl, dev, dom, args.warmup, args.steps)
+    cycles = args.steps
+    zcps = E * cycles / elapsed
+    print(f"[bench] fp64: {cycles} cycles in {elapsed:.3f}s -> {zcps:.4e} zc/s",
+          file=sys.stderr, flush=True)
+    dev.download_view()
+    e0_fp64 = float(dom.field("e")[0])
+    rep64 = dom.report()
+
+    # per-kernel HIP-event timings over a few more cycles
+    dev.set_timing(True)
+    ts2 = dom.time_state()
+    ts2.time, ts2.deltatime, ts2.dtcourant, ts2.dthydro, ts2.cycle = (
+        ts.time, ts.deltatime, ts.dtcourant, ts.dthydro, ts.cycle)
+    prof_cycles = 8
+    rc = dev.run(ts.cycle + prof_cycles, ts2)
+    assert rc == 0, rc
+    kt = dev.kernel_times()
+    dev.set_timing(False)
+    kernels = {}
+    for name, (avg_ms, n) in kt.items():
+        if n == 0:
+            continue
+        b = kernel_bytes(name, E, N)
+        per_cycle = n / prof_cycles
+        kernels[name] = {"avg_ms": round(avg_ms, 5), "launches_per_cycle": per_cycle,
+                         "bytes_per_launch": b / max(per_cycle, 1),
+                         "GBs": round((b / max(per_cycle, 1)) / (avg_ms * 1e-3) / 1e9, 1)}
+    dom_k = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches_per_cycle"])
+    dk = kernels[dom_k]
+    achieved = dk["bytes_per_launch"] / (dk["avg_ms"] * 1e-3) / 1e9
+    ms_cycle_events = sum(k["avg_ms"] * k["launches_per_cycle"] for k in kernels.values())
+    dev.close()
+
+    # ---- FP32-mixed form -------------------------------------------------
+    mixed = None
+    if not args.no_mixed:
+        domm = pyl.Domain(s, nr)
+        devm = pyl.Device(domm, pyl.MIXED)
+        el_m, tsm = timed_run(pyl, devm, domm, args.warmup, args.steps)
+        devm.download_view()
+        e0m = float(domm.field("e")[0])
+        mixed = {"value": E * args.steps / el_m, "ms_per_step": 1e3 * el_m / args.steps,
+                 "origin_energy_rel_err_vs_fp64": abs(e0m - e0_fp64) / abs(e0_fp64),
+                 "cycles": tsm.cycle}
+        devm.close()
+        domm.close()
+
+    # ---- CPU baseline ----------------------------------------------------
+    cpu = None
+    if not args.no_cpu:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        cs = min(s, 150)
+        v, ncyc, el = cpu_baseline(cs, nr, args.cpu_seconds, threads)
+        cpu = {"value": v, "unit": "zone-cycles/s", "cores": threads, "kind": "port",
+               "sample": f"OpenMP oracle, s={cs}, {nr} regions, {ncyc} cycles after cycle 1, "
+                         f"{el:.1f}s wall, gcc -O3 -ffp-contract=off"}
+
+    bsz = model_bytes_per_zone_cycle(s)
+    out = {
+        "metric": "zone-cycles/sec FP64 s=150 vs FP32-mixed & 64-problem batch; % HBM peak",
+        "value": zcps,
+        "unit": "zone-cycles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / cycles,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (deterministic Sedov initial conditions)",
+        "config": {"workload": f"LULESH Sedov s={s} ({E} elements, {N} nodes), {nr} regions, "
+                               "-b 1 -c 1, FP64, cycles from the start of the run",
+                   "s": s, "regions": nr, "elements": E},
+        "roofline": {"bound": "hbm", "kernel": dom_k, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "model_bytes_per_zone_cycle": round(bsz, 1),
+        "model_hbm_frac": round(bsz * zcps / (HBM_PEAK_GBS * 1e9), 4),
+        "kernels": kernels,
+        "ms_per_cycle_kernel_sum": round(ms_cycle_events, 4),
+        "origin_energy": e0_fp64,
+        "symmetry_max_abs_diff": float(rep64[1]),
+        "mixed": mixed,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (zcps / cpu["value"]) if cpu else None,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

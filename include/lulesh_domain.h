@@ -1,0 +1,52 @@
+/*
+ * lulesh_domain.h -- C-ABI of the host side: Domain construction, the host
+ * TimeIncrement and the final report.
+ *
+ * The C++ class behind the opaque handle keeps the reference Domain's
+ * accessor names (lulesh.h:130-606); these entry points let a non-C++ caller
+ * (the Python test and bench harness, or any FFI) build and inspect it.
+ */
+#ifndef LULESH_DOMAIN_H
+#define LULESH_DOMAIN_H
+
+#include <stdint.h>
+
+#include "lulesh_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lulesh_domain lulesh_domain;
+
+/* new Domain(numRanks=1, col=row=plane=0, nx, tp=1, nr, balance, cost):
+ * replaces lulesh.cc:2364 -> lulesh-init.cc:10-183.  NULL on bad input. */
+lulesh_domain *lulesh_domain_create(int32_t nx, int32_t nr, int32_t balance, int32_t cost);
+void lulesh_domain_destroy(lulesh_domain *d);
+
+/* Fill a plain-pointer view of the Domain's live arrays (for lulesh_hip_create). */
+int lulesh_domain_get_view(lulesh_domain *d, lulesh_domain_view *out);
+/* Copy the time state of a view back into the Domain (time, deltatime,
+ * dtcourant, dthydro, cycle). */
+int lulesh_domain_set_time_state(lulesh_domain *d, const lulesh_time_state *ts);
+int lulesh_domain_get_time_state(lulesh_domain *d, lulesh_time_state *ts);
+
+/* TimeIncrement, lulesh.cc:182-229, on the host Domain. */
+void lulesh_time_increment(lulesh_domain *d);
+
+/* LagrangeLeapFrog(Domain&): the cycle body of lulesh.cc:2378 run through the
+ * device context (lulesh_hip_leapfrog with the Domain's deltatime); stores
+ * dtcourant / dthydro into the Domain. */
+int lulesh_lagrange_leapfrog(lulesh_domain *d, lulesh_ctx *ctx);
+
+/* VerifyAndWriteFinalOutput numbers, lulesh-util.cc:60-97:
+ * out = {e(0), MaxAbsDiff, TotalAbsDiff, MaxRelDiff}. */
+void lulesh_verify_report(lulesh_domain *d, double out[4]);
+
+/* Region repetition count of the EOS (lulesh.cc:2159-2169). */
+int32_t lulesh_region_rep(int32_t reg, int32_t numReg, int32_t cost);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,164 @@
+/*
+ * lulesh_hip.h -- C-ABI of the MI355X (gfx950) LULESH Lagrange leapfrog path.
+ *
+ * This is the drop-in boundary for the reference's hot path.  In the
+ * reference, hpx_main's cycle loop calls
+ *
+ *     TimeIncrement(*locDom);               // lulesh.cc:2377  (lulesh.cc:182-229)
+ *     LagrangeLeapFrogWithTasks(*locDom);   // lulesh.cc:2378  (lulesh.cc:1977-2240)
+ *
+ * on a Domain whose state lives in host std::vectors (lulesh.h:130-606).
+ * Here the Domain's SoA fields are copied once into HBM by lulesh_hip_create
+ * and advanced there; the host Domain is refreshed by lulesh_hip_download.
+ *
+ * Conventions (kept from the reference):
+ *   - return 0 on success; -1 = VolumeError, -2 = QStopError (lulesh.h:30,
+ *     where the reference calls exit(VolumeError) at lulesh.cc:821, 908, 1262);
+ *     the host program turns a negative code into exit(code) so the process
+ *     status matches the reference.  Codes <= -100 wrap HIP runtime errors,
+ *     -3 = invalid argument.
+ *   - inputs are copied at create; the caller keeps ownership of its arrays.
+ *   - a context is not thread-safe: one host thread and one HIP stream each.
+ *   - all device memory is allocated once at create (the reference mallocs
+ *     corner and gradient buffers every cycle, lulesh.cc:2017-2022, 2117).
+ *
+ * Only plain C types appear in this header.
+ */
+#ifndef LULESH_HIP_H
+#define LULESH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes, lulesh.h:30 */
+#define LULESH_OK 0
+#define LULESH_VOLUME_ERROR (-1)
+#define LULESH_QSTOP_ERROR (-2)
+#define LULESH_INVALID_ARGUMENT (-3)
+#define LULESH_HIP_ERROR (-100)
+
+/* Arithmetic forms. */
+#define LULESH_PRECISION_FP64 0  /* every kernel in FP64 (the reference's Real_t, lulesh.h:27) */
+#define LULESH_PRECISION_MIXED 1 /* hourglass + monotonic-Q gradients in FP32, the rest FP64 */
+
+/*
+ * Plain-pointer view of a Domain (lulesh.h:130-606).  Array lengths:
+ * N = numNode, E = numElem, P = (sizeX+1)^2.  Pointers to fields that the
+ * path recomputes before reading (fx.., xdd.., delv_*.., vnew) may be NULL
+ * at create.
+ */
+typedef struct lulesh_domain_view {
+    int32_t sizeX;   /* edge elements (s): lulesh.h:439 */
+    int32_t numElem; /* s^3 */
+    int32_t numNode; /* (s+1)^3 */
+    int32_t numReg;  /* lulesh.h:490 */
+    int32_t cost;    /* lulesh.h:491 (-c) */
+    int32_t pad0;
+    /* node-centred [N], lulesh.h:464-481 */
+    double *x, *y, *z, *xd, *yd, *zd, *xdd, *ydd, *zdd, *fx, *fy, *fz, *nodalMass;
+    /* symmetry node sets [P], lulesh.h:483-485 */
+    int32_t *symmX, *symmY, *symmZ;
+    /* element-centred, lulesh.h:496-536 */
+    int32_t *nodelist; /* [8E] */
+    int32_t *lxim, *lxip, *letam, *letap, *lzetam, *lzetap, *elemBC; /* [E] */
+    double *e, *p, *q, *ql, *qq, *v, *volo, *delv, *vdov, *arealg, *ss, *elemMass, *vnew;
+    double *delv_xi, *delv_eta, *delv_zeta, *delx_xi, *delx_eta, *delx_zeta;
+    /* regions, lulesh.h:492-494 */
+    int32_t *regNumList;   /* [E], region number + 1 */
+    int32_t *regElemSize;  /* [numReg] */
+    int32_t **regElemlist; /* [numReg][regElemSize[r]] */
+    /* corner CSR, lulesh.h:592-593 */
+    int32_t *nodeElemStart;      /* [N+1] */
+    int32_t *nodeElemCornerList; /* [8E] */
+    /* constants, lulesh.h:539-560 */
+    double e_cut, p_cut, q_cut, v_cut, u_cut, hgcoef, ss4o3, qstop, monoq_max_slope,
+        monoq_limiter_mult, qlc_monoq, qqc_monoq, qqc, eosvmax, eosvmin, pmin, emin, dvovmax,
+        refdens;
+    /* time state, lulesh.h:563-572 */
+    double dtcourant, dthydro, dtfixed, time, deltatime, deltatimemultlb, deltatimemultub,
+        dtmax, stoptime;
+    int32_t cycle;
+    int32_t pad1;
+} lulesh_domain_view;
+
+/* Time state advanced by the device-side TimeIncrement (lulesh.cc:182-229). */
+typedef struct lulesh_time_state {
+    double time, deltatime, dtcourant, dthydro;
+    int32_t cycle;
+    int32_t status; /* 0, or the error code that stopped the loop */
+} lulesh_time_state;
+
+/* Field identifiers for upload/download. */
+enum lulesh_field_id {
+    LULESH_F_X = 0, LULESH_F_Y, LULESH_F_Z, LULESH_F_XD, LULESH_F_YD, LULESH_F_ZD,
+    LULESH_F_XDD, LULESH_F_YDD, LULESH_F_ZDD, LULESH_F_FX, LULESH_F_FY, LULESH_F_FZ,
+    LULESH_F_NODALMASS,
+    LULESH_F_E, LULESH_F_P, LULESH_F_Q, LULESH_F_QL, LULESH_F_QQ, LULESH_F_V, LULESH_F_VOLO,
+    LULESH_F_DELV, LULESH_F_VDOV, LULESH_F_AREALG, LULESH_F_SS, LULESH_F_ELEMMASS,
+    LULESH_F_VNEW, LULESH_F_DELV_XI, LULESH_F_DELV_ETA, LULESH_F_DELV_ZETA, LULESH_F_DELX_XI,
+    LULESH_F_DELX_ETA, LULESH_F_DELX_ZETA,
+    LULESH_F_COUNT
+};
+
+/* Phases of the cycle, for phase-level parity checks. */
+enum lulesh_phase_id {
+    LULESH_PHASE_NODAL = 0,    /* lulesh.cc:2017-2110: forces, gather, accel, BC, vel, pos */
+    LULESH_PHASE_ELEMENTS = 1, /* lulesh.cc:2112-2136: kinematics, volumes, Q gradients */
+    LULESH_PHASE_MATERIAL = 2, /* lulesh.cc:2138-2202: monotonic Q, EOS x rep, sound speed */
+    LULESH_PHASE_CONSTRAINTS = 3 /* lulesh.cc:2204-2239: dtcourant / dthydro */
+};
+
+typedef struct lulesh_ctx lulesh_ctx;
+
+/* Copy a Domain into HBM.  Replaces the per-cycle allocations of
+ * LagrangeLeapFrogWithTasks (lulesh.cc:2017-2022, 2117) with one allocation. */
+int lulesh_hip_create(const lulesh_domain_view *host, int precision, lulesh_ctx **out);
+
+/* One Lagrange leapfrog cycle with a host-chosen time step: the drop-in for
+ * LagrangeLeapFrogWithTasks(domain) (lulesh.cc:1977-2240), called after the
+ * host's TimeIncrement (lulesh.cc:2377).  On return *dtcourant / *dthydro hold
+ * the constraint minima (lulesh.cc:2234-2235). */
+int lulesh_hip_leapfrog(lulesh_ctx *ctx, double deltatime, double *dtcourant,
+                        double *dthydro);
+
+/* The reference cycle loop (lulesh.cc:2374-2386) run on the device: repeat
+ * {TimeIncrement; LagrangeLeapFrog} while time < stoptime and cycle <
+ * max_cycles.  *inout supplies and returns the time state.  Cycles are
+ * replayed from captured HIP graphs; TimeIncrement runs on the device. */
+int lulesh_hip_run(lulesh_ctx *ctx, int32_t max_cycles, lulesh_time_state *inout);
+
+/* One phase of the cycle (see lulesh_phase_id) with time step deltatime.
+ * The MATERIAL and CONSTRAINTS phases restart the constraint minima at 1e20
+ * (lulesh.cc:2211-2212); lulesh_hip_constraints_result reads them back. */
+int lulesh_hip_phase(lulesh_ctx *ctx, int phase, double deltatime);
+int lulesh_hip_constraints_result(lulesh_ctx *ctx, double *dtcourant, double *dthydro);
+
+/* Synchronous copies between HBM and host arrays of `count` elements. */
+int lulesh_hip_download(lulesh_ctx *ctx, int field_id, void *dst, size_t count);
+int lulesh_hip_upload(lulesh_ctx *ctx, int field_id, const void *src, size_t count);
+/* Download every state field into a view (NULL pointers are skipped), and
+ * the time state. */
+int lulesh_hip_download_view(lulesh_ctx *ctx, lulesh_domain_view *dst);
+
+/* Kernel timing of the last lulesh_hip_run call that had timing enabled:
+ * names[i] / avg_ms[i] / launches[i] for up to `cap` kernels; returns the
+ * number of kernels.  Timing brackets every launch with HIP events on the
+ * context's stream. */
+int lulesh_hip_set_timing(lulesh_ctx *ctx, int enable);
+int lulesh_hip_kernel_times(lulesh_ctx *ctx, const char **names, double *avg_ms,
+                            int64_t *launches, int cap);
+
+/* Device memory held by the context, in bytes. */
+int64_t lulesh_hip_device_bytes(const lulesh_ctx *ctx);
+
+const char *lulesh_hip_last_error(const lulesh_ctx *ctx);
+void lulesh_hip_destroy(lulesh_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,496 @@
+// Element- and node-level device math of the LULESH Lagrange leapfrog cycle
+// for gfx950.
+//
+// Every function keeps the expression shape of the reference routine it
+// names (file:line given at each), and the library is compiled with
+// -ffp-contract=off, so in FP64 each multiply and add is the same IEEE
+// operation as on the host and results can be compared bit for bit.
+// Functions templated on T run in FP32 in the mixed-precision form.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lulesh_dev {
+
+// Boundary-condition bits, lulesh.h:47-75
+enum : int32_t {
+    XI_M = 0x00007, XI_M_SYMM = 0x00001, XI_M_FREE = 0x00002, XI_M_COMM = 0x00004,
+    XI_P = 0x00038, XI_P_SYMM = 0x00008, XI_P_FREE = 0x00010, XI_P_COMM = 0x00020,
+    ETA_M = 0x001c0, ETA_M_SYMM = 0x00040, ETA_M_FREE = 0x00080, ETA_M_COMM = 0x00100,
+    ETA_P = 0x00e00, ETA_P_SYMM = 0x00200, ETA_P_FREE = 0x00400, ETA_P_COMM = 0x00800,
+    ZETA_M = 0x07000, ZETA_M_SYMM = 0x01000, ZETA_M_FREE = 0x02000, ZETA_M_COMM = 0x04000,
+    ZETA_P = 0x38000, ZETA_P_SYMM = 0x08000, ZETA_P_FREE = 0x10000, ZETA_P_COMM = 0x20000,
+};
+
+// Run constants (lulesh-init.cc:18-36) passed by value to the kernels.
+struct Params {
+    double e_cut, p_cut, q_cut, v_cut, u_cut, hgcoef, monoq_max_slope, monoq_limiter_mult,
+        qlc_monoq, qqc_monoq, qqc, eosvmax, eosvmin, pmin, emin, dvovmax, refdens;
+};
+
+__device__ __forceinline__ double sqrt_(double a) { return __builtin_sqrt(a); }
+__device__ __forceinline__ float sqrt_(float a) { return __builtin_sqrtf(a); }
+__device__ __forceinline__ double cbrt_(double a) { return ::cbrt(a); }
+__device__ __forceinline__ float cbrt_(float a) { return ::cbrtf(a); }
+__device__ __forceinline__ double fabs_(double a) { return __builtin_fabs(a); }
+
+// Keep the compiler from treating one repetition of an idempotent loop body
+// as equal to the previous one: the value is declared modified by an empty
+// asm statement (no instruction is emitted).
+__device__ __forceinline__ void opaque(double &v) { asm volatile("" : "+v"(v)); }
+
+// CalcElemVolume, lulesh.cc:592-674
+template <typename T>
+__device__ __forceinline__ T elem_volume(const T x[8], const T y[8], const T z[8]) {
+    const T twelveth = T(1.0) / T(12.0);
+    const T dx61 = x[6] - x[1], dy61 = y[6] - y[1], dz61 = z[6] - z[1];
+    const T dx70 = x[7] - x[0], dy70 = y[7] - y[0], dz70 = z[7] - z[0];
+    const T dx63 = x[6] - x[3], dy63 = y[6] - y[3], dz63 = z[6] - z[3];
+    const T dx20 = x[2] - x[0], dy20 = y[2] - y[0], dz20 = z[2] - z[0];
+    const T dx50 = x[5] - x[0], dy50 = y[5] - y[0], dz50 = z[5] - z[0];
+    const T dx64 = x[6] - x[4], dy64 = y[6] - y[4], dz64 = z[6] - z[4];
+    const T dx31 = x[3] - x[1], dy31 = y[3] - y[1], dz31 = z[3] - z[1];
+    const T dx72 = x[7] - x[2], dy72 = y[7] - y[2], dz72 = z[7] - z[2];
+    const T dx43 = x[4] - x[3], dy43 = y[4] - y[3], dz43 = z[4] - z[3];
+    const T dx57 = x[5] - x[7], dy57 = y[5] - y[7], dz57 = z[5] - z[7];
+    const T dx14 = x[1] - x[4], dy14 = y[1] - y[4], dz14 = z[1] - z[4];
+    const T dx25 = x[2] - x[5], dy25 = y[2] - y[5], dz25 = z[2] - z[5];
+    // grouping of the reference's TRIPLE_PRODUCT macro, lulesh.cc:649-651
+    auto triple = [](T x1, T y1, T z1, T x2, T y2, T z2, T x3, T y3, T z3) {
+        return x1 * (y2 * z3 - z2 * y3) + x2 * (z1 * y3 - y1 * z3) + x3 * (y1 * z2 - z1 * y2);
+    };
+    T volume = triple(dx31 + dx72, dx63, dx20, dy31 + dy72, dy63, dy20, dz31 + dz72, dz63, dz20) +
+               triple(dx43 + dx57, dx64, dx70, dy43 + dy57, dy64, dy70, dz43 + dz57, dz64, dz70) +
+               triple(dx14 + dx25, dx61, dx50, dy14 + dy25, dy61, dy50, dz14 + dz25, dz61, dz50);
+    volume *= twelveth;
+    return volume;
+}
+
+// Jacobian columns and cofactors at the element centre,
+// CalcElemShapeFunctionDerivatives, lulesh.cc:277-374.  b may be null when
+// only the determinant is needed (the stress path overwrites b with the
+// node normals, lulesh.cc:817-824).
+template <bool WANT_B>
+__device__ __forceinline__ double shape_derivatives(const double x[8], const double y[8],
+                                                    const double z[8], double b[3][8]) {
+    const double fjxxi = .125 * ((x[6] - x[0]) + (x[5] - x[3]) - (x[7] - x[1]) - (x[4] - x[2]));
+    const double fjxet = .125 * ((x[6] - x[0]) - (x[5] - x[3]) + (x[7] - x[1]) - (x[4] - x[2]));
+    const double fjxze = .125 * ((x[6] - x[0]) + (x[5] - x[3]) + (x[7] - x[1]) + (x[4] - x[2]));
+    const double fjyxi = .125 * ((y[6] - y[0]) + (y[5] - y[3]) - (y[7] - y[1]) - (y[4] - y[2]));
+    const double fjyet = .125 * ((y[6] - y[0]) - (y[5] - y[3]) + (y[7] - y[1]) - (y[4] - y[2]));
+    const double fjyze = .125 * ((y[6] - y[0]) + (y[5] - y[3]) + (y[7] - y[1]) + (y[4] - y[2]));
+    const double fjzxi = .125 * ((z[6] - z[0]) + (z[5] - z[3]) - (z[7] - z[1]) - (z[4] - z[2]));
+    const double fjzet = .125 * ((z[6] - z[0]) - (z[5] - z[3]) + (z[7] - z[1]) - (z[4] - z[2]));
+    const double fjzze = .125 * ((z[6] - z[0]) + (z[5] - z[3]) + (z[7] - z[1]) + (z[4] - z[2]));
+    const double cjxet = -(fjyxi * fjzze) + (fjzxi * fjyze);
+    const double cjyet = (fjxxi * fjzze) - (fjzxi * fjxze);
+    const double cjzet = -(fjxxi * fjyze) + (fjyxi * fjxze);
+    if (WANT_B) {
+        const double cjxxi = (fjyet * fjzze) - (fjzet * fjyze);
+        const double cjxze = (fjyxi * fjzet) - (fjzxi * fjyet);
+        const double cjyxi = -(fjxet * fjzze) + (fjzet * fjxze);
+        const double cjyze = -(fjxxi * fjzet) + (fjzxi * fjxet);
+        const double cjzxi = (fjxet * fjyze) - (fjyet * fjxze);
+        const double cjzze = (fjxxi * fjyet) - (fjyxi * fjxet);
+        b[0][0] = -cjxxi - cjxet - cjxze;
+        b[0][1] = cjxxi - cjxet - cjxze;
+        b[0][2] = cjxxi + cjxet - cjxze;
+        b[0][3] = -cjxxi + cjxet - cjxze;
+        b[0][4] = -b[0][2];
+        b[0][5] = -b[0][3];
+        b[0][6] = -b[0][0];
+        b[0][7] = -b[0][1];
+        b[1][0] = -cjyxi - cjyet - cjyze;
+        b[1][1] = cjyxi - cjyet - cjyze;
+        b[1][2] = cjyxi + cjyet - cjyze;
+        b[1][3] = -cjyxi + cjyet - cjyze;
+        b[1][4] = -b[1][2];
+        b[1][5] = -b[1][3];
+        b[1][6] = -b[1][0];
+        b[1][7] = -b[1][1];
+        b[2][0] = -cjzxi - cjzet - cjzze;
+        b[2][1] = cjzxi - cjzet - cjzze;
+        b[2][2] = cjzxi + cjzet - cjzze;
+        b[2][3] = -cjzxi + cjzet - cjzze;
+        b[2][4] = -b[2][2];
+        b[2][5] = -b[2][3];
+        b[2][6] = -b[2][0];
+        b[2][7] = -b[2][1];
+    }
+    return 8. * (fjxet * cjxet + fjyet * cjyet + fjzet * cjzet);
+}
+
+// SumElemFaceNormal, lulesh.cc:378-409: one quarter of the face's area
+// vector (from the two bisectors) is added to each of its four nodes.
+__device__ __forceinline__ void face_normal(double pf[3][8], int n0, int n1, int n2, int n3,
+                                            const double x[8], const double y[8],
+                                            const double z[8]) {
+    const double bx0 = 0.5 * (x[n3] + x[n2] - x[n1] - x[n0]);
+    const double by0 = 0.5 * (y[n3] + y[n2] - y[n1] - y[n0]);
+    const double bz0 = 0.5 * (z[n3] + z[n2] - z[n1] - z[n0]);
+    const double bx1 = 0.5 * (x[n2] + x[n1] - x[n3] - x[n0]);
+    const double by1 = 0.5 * (y[n2] + y[n1] - y[n3] - y[n0]);
+    const double bz1 = 0.5 * (z[n2] + z[n1] - z[n3] - z[n0]);
+    const double ax = 0.25 * (by0 * bz1 - bz0 * by1);
+    const double ay = 0.25 * (bz0 * bx1 - bx0 * bz1);
+    const double az = 0.25 * (bx0 * by1 - by0 * bx1);
+    pf[0][n0] += ax; pf[0][n1] += ax; pf[0][n2] += ax; pf[0][n3] += ax;
+    pf[1][n0] += ay; pf[1][n1] += ay; pf[1][n2] += ay; pf[1][n3] += ay;
+    pf[2][n0] += az; pf[2][n1] += az; pf[2][n2] += az; pf[2][n3] += az;
+}
+
+// CalcElemNodeNormals, lulesh.cc:413-451 (the face order fixes the rounding
+// of each node's three-term sum).
+__device__ __forceinline__ void node_normals(double pf[3][8], const double x[8],
+                                             const double y[8], const double z[8]) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) pf[0][i] = pf[1][i] = pf[2][i] = 0.0;
+    face_normal(pf, 0, 1, 2, 3, x, y, z);
+    face_normal(pf, 0, 4, 5, 1, x, y, z);
+    face_normal(pf, 1, 5, 6, 2, x, y, z);
+    face_normal(pf, 2, 6, 7, 3, x, y, z);
+    face_normal(pf, 3, 7, 4, 0, x, y, z);
+    face_normal(pf, 4, 7, 6, 5, x, y, z);
+}
+
+// VoluDer, lulesh.cc:468-491
+template <typename T>
+__device__ __forceinline__ void volu_der(T x0, T x1, T x2, T x3, T x4, T x5, T y0, T y1, T y2,
+                                         T y3, T y4, T y5, T z0, T z1, T z2, T z3, T z4, T z5,
+                                         T &dvdx, T &dvdy, T &dvdz) {
+    const T twelfth = T(1.0) / T(12.0);
+    dvdx = (y1 + y2) * (z0 + z1) - (y0 + y1) * (z1 + z2) + (y0 + y4) * (z3 + z4) -
+           (y3 + y4) * (z0 + z4) - (y2 + y5) * (z3 + z5) + (y3 + y5) * (z2 + z5);
+    dvdy = -(x1 + x2) * (z0 + z1) + (x0 + x1) * (z1 + z2) - (x0 + x4) * (z3 + z4) +
+           (x3 + x4) * (z0 + z4) + (x2 + x5) * (z3 + z5) - (x3 + x5) * (z2 + z5);
+    dvdz = -(y1 + y2) * (x0 + x1) + (y0 + y1) * (x1 + x2) - (y0 + y4) * (x3 + x4) +
+           (y3 + y4) * (x0 + x4) + (y2 + y5) * (x3 + x5) - (y3 + y5) * (x2 + x5);
+    dvdx *= twelfth;
+    dvdy *= twelfth;
+    dvdz *= twelfth;
+}
+
+// CalcElemVolumeDerivative, lulesh.cc:495-523
+template <typename T>
+__device__ __forceinline__ void volume_derivative(T dvdx[8], T dvdy[8], T dvdz[8], const T x[8],
+                                                  const T y[8], const T z[8]) {
+#define LULESH_VD(a, b, c, d, e, f, o)                                                       \
+    volu_der<T>(x[a], x[b], x[c], x[d], x[e], x[f], y[a], y[b], y[c], y[d], y[e], y[f], z[a], \
+                z[b], z[c], z[d], z[e], z[f], dvdx[o], dvdy[o], dvdz[o])
+    LULESH_VD(1, 2, 3, 4, 5, 7, 0);
+    LULESH_VD(0, 1, 2, 7, 4, 6, 3);
+    LULESH_VD(3, 0, 1, 6, 7, 5, 2);
+    LULESH_VD(2, 3, 0, 5, 6, 4, 1);
+    LULESH_VD(7, 6, 5, 0, 3, 1, 4);
+    LULESH_VD(4, 7, 6, 1, 0, 2, 5);
+    LULESH_VD(5, 4, 7, 2, 1, 3, 6);
+    LULESH_VD(6, 5, 4, 3, 2, 0, 7);
+#undef LULESH_VD
+}
+
+// Hourglass base vectors, lulesh.cc:921-954 (entries are +-1).
+__device__ __forceinline__ constexpr double gamma_hg(int mode, int node) {
+    constexpr signed char g[4][8] = {{1, 1, -1, -1, -1, -1, 1, 1},
+                                     {1, -1, -1, 1, -1, 1, 1, -1},
+                                     {1, -1, 1, -1, 1, -1, 1, -1},
+                                     {-1, 1, -1, 1, 1, -1, 1, -1}};
+    return double(g[mode][node]);
+}
+
+// Flanagan-Belytschko hourglass force of one element: the hourglass modes
+// (lulesh.cc:958-1023), coefficient (1028-1068) and
+// CalcElemFBHourglassForce (527-562).  T = double (FP64 form) or float
+// (mixed form: coordinates are shifted to node 0 in FP64 before rounding so
+// that the FP32 differences keep their significant bits).
+template <typename T>
+__device__ __forceinline__ void hourglass_force(const T x[8], const T y[8], const T z[8],
+                                                const T xd[8], const T yd[8], const T zd[8],
+                                                T determ, T coefficient, T hgf[3][8]) {
+    T dvdx[8], dvdy[8], dvdz[8];
+    volume_derivative<T>(dvdx, dvdy, dvdz, x, y, z);
+    const T volinv = T(1.0) / determ;
+    T hourgam[8][4];
+#pragma unroll
+    for (int i1 = 0; i1 < 4; ++i1) {
+        T hx = x[0] * T(gamma_hg(i1, 0)), hy = y[0] * T(gamma_hg(i1, 0)),
+          hz = z[0] * T(gamma_hg(i1, 0));
+#pragma unroll
+        for (int j = 1; j < 8; ++j) {
+            hx = hx + x[j] * T(gamma_hg(i1, j));
+            hy = hy + y[j] * T(gamma_hg(i1, j));
+            hz = hz + z[j] * T(gamma_hg(i1, j));
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            hourgam[j][i1] =
+                T(gamma_hg(i1, j)) - volinv * (dvdx[j] * hx + dvdy[j] * hy + dvdz[j] * hz);
+    }
+    const T *vel[3] = {xd, yd, zd};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        T hxx[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            T s = hourgam[0][i] * vel[c][0];
+#pragma unroll
+            for (int j = 1; j < 8; ++j) s = s + hourgam[j][i] * vel[c][j];
+            hxx[i] = s;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            hgf[c][i] = coefficient * (hourgam[i][0] * hxx[0] + hourgam[i][1] * hxx[1] +
+                                       hourgam[i][2] * hxx[2] + hourgam[i][3] * hxx[3]);
+    }
+}
+
+// AreaFace, lulesh.cc:678-692
+__device__ __forceinline__ double area_face(double x0, double x1, double x2, double x3, double y0,
+                                            double y1, double y2, double y3, double z0, double z1,
+                                            double z2, double z3) {
+    const double fx = (x2 - x0) - (x3 - x1), fy = (y2 - y0) - (y3 - y1), fz = (z2 - z0) - (z3 - z1);
+    const double gx = (x2 - x0) + (x3 - x1), gy = (y2 - y0) + (y3 - y1), gz = (z2 - z0) + (z3 - z1);
+    return (fx * fx + fy * fy + fz * fz) * (gx * gx + gy * gy + gz * gz) -
+           (fx * gx + fy * gy + fz * gz) * (fx * gx + fy * gy + fz * gz);
+}
+
+// CalcElemCharacteristicLength, lulesh.cc:696-729 (std::max(a,b) = a<b ? b : a)
+__device__ __forceinline__ double characteristic_length(const double x[8], const double y[8],
+                                                        const double z[8], double volume) {
+    double cl = 0.0, a;
+#define LULESH_AF(a0, a1, a2, a3)                                                             \
+    a = area_face(x[a0], x[a1], x[a2], x[a3], y[a0], y[a1], y[a2], y[a3], z[a0], z[a1], z[a2], \
+                  z[a3]);                                                                     \
+    cl = (a < cl) ? cl : a;
+    LULESH_AF(0, 1, 2, 3)
+    LULESH_AF(4, 5, 6, 7)
+    LULESH_AF(0, 1, 5, 4)
+    LULESH_AF(1, 2, 6, 5)
+    LULESH_AF(2, 3, 7, 6)
+    LULESH_AF(3, 0, 4, 7)
+#undef LULESH_AF
+    return 4.0 * volume / sqrt_(cl);
+}
+
+// Diagonal of CalcElemVelocityGradient (lulesh.cc:733-753): only the trace
+// survives (vdov, lulesh.cc:1239); the off-diagonal terms and the deviatoric
+// strains are dead in the reference.
+__device__ __forceinline__ double velocity_divergence(const double xv[8], const double yv[8],
+                                                      const double zv[8], const double b[3][8],
+                                                      double detJ) {
+    const double inv = 1.0 / detJ;
+#define LULESH_GRAD(pf, v)                                                                    \
+    (inv * (pf[0] * (v[0] - v[6]) + pf[1] * (v[1] - v[7]) + pf[2] * (v[2] - v[4]) +            \
+            pf[3] * (v[3] - v[5])))
+    const double dxx = LULESH_GRAD(b[0], xv);
+    const double dyy = LULESH_GRAD(b[1], yv);
+    const double dzz = LULESH_GRAD(b[2], zv);
+#undef LULESH_GRAD
+    return dxx + dyy + dzz;
+}
+
+// Monotonic-Q gradients of one element, lulesh.cc:1349-1416.
+// out: delx_xi, delx_eta, delx_zeta, delv_xi, delv_eta, delv_zeta
+template <typename T>
+__device__ __forceinline__ void q_gradients(const T x[8], const T y[8], const T z[8],
+                                            const T xv[8], const T yv[8], const T zv[8], T vol,
+                                            T out[6]) {
+    const T ptiny = T(1.e-36);
+    const T norm = T(1.0) / (vol + ptiny);
+    const T dxj = T(-0.25) * ((x[0] + x[1] + x[5] + x[4]) - (x[3] + x[2] + x[6] + x[7]));
+    const T dyj = T(-0.25) * ((y[0] + y[1] + y[5] + y[4]) - (y[3] + y[2] + y[6] + y[7]));
+    const T dzj = T(-0.25) * ((z[0] + z[1] + z[5] + z[4]) - (z[3] + z[2] + z[6] + z[7]));
+    const T dxi = T(0.25) * ((x[1] + x[2] + x[6] + x[5]) - (x[0] + x[3] + x[7] + x[4]));
+    const T dyi = T(0.25) * ((y[1] + y[2] + y[6] + y[5]) - (y[0] + y[3] + y[7] + y[4]));
+    const T dzi = T(0.25) * ((z[1] + z[2] + z[6] + z[5]) - (z[0] + z[3] + z[7] + z[4]));
+    const T dxk = T(0.25) * ((x[4] + x[5] + x[6] + x[7]) - (x[0] + x[1] + x[2] + x[3]));
+    const T dyk = T(0.25) * ((y[4] + y[5] + y[6] + y[7]) - (y[0] + y[1] + y[2] + y[3]));
+    const T dzk = T(0.25) * ((z[4] + z[5] + z[6] + z[7]) - (z[0] + z[1] + z[2] + z[3]));
+    T ax, ay, az, dxv, dyv, dzv;
+    // zeta: i cross j
+    ax = dyi * dzj - dzi * dyj;
+    ay = dzi * dxj - dxi * dzj;
+    az = dxi * dyj - dyi * dxj;
+    out[2] = vol / sqrt_(ax * ax + ay * ay + az * az + ptiny);
+    ax *= norm; ay *= norm; az *= norm;
+    dxv = T(0.25) * ((xv[4] + xv[5] + xv[6] + xv[7]) - (xv[0] + xv[1] + xv[2] + xv[3]));
+    dyv = T(0.25) * ((yv[4] + yv[5] + yv[6] + yv[7]) - (yv[0] + yv[1] + yv[2] + yv[3]));
+    dzv = T(0.25) * ((zv[4] + zv[5] + zv[6] + zv[7]) - (zv[0] + zv[1] + zv[2] + zv[3]));
+    out[5] = ax * dxv + ay * dyv + az * dzv;
+    // xi: j cross k
+    ax = dyj * dzk - dzj * dyk;
+    ay = dzj * dxk - dxj * dzk;
+    az = dxj * dyk - dyj * dxk;
+    out[0] = vol / sqrt_(ax * ax + ay * ay + az * az + ptiny);
+    ax *= norm; ay *= norm; az *= norm;
+    dxv = T(0.25) * ((xv[1] + xv[2] + xv[6] + xv[5]) - (xv[0] + xv[3] + xv[7] + xv[4]));
+    dyv = T(0.25) * ((yv[1] + yv[2] + yv[6] + yv[5]) - (yv[0] + yv[3] + yv[7] + yv[4]));
+    dzv = T(0.25) * ((zv[1] + zv[2] + zv[6] + zv[5]) - (zv[0] + zv[3] + zv[7] + zv[4]));
+    out[3] = ax * dxv + ay * dyv + az * dzv;
+    // eta: k cross i
+    ax = dyk * dzi - dzk * dyi;
+    ay = dzk * dxi - dxk * dzi;
+    az = dxk * dyi - dyk * dxi;
+    out[1] = vol / sqrt_(ax * ax + ay * ay + az * az + ptiny);
+    ax *= norm; ay *= norm; az *= norm;
+    dxv = T(-0.25) * ((xv[0] + xv[1] + xv[5] + xv[4]) - (xv[3] + xv[2] + xv[6] + xv[7]));
+    dyv = T(-0.25) * ((yv[0] + yv[1] + yv[5] + yv[4]) - (yv[3] + yv[2] + yv[6] + yv[7]));
+    dzv = T(-0.25) * ((zv[0] + zv[1] + zv[5] + zv[4]) - (zv[3] + zv[2] + zv[6] + zv[7]));
+    out[4] = ax * dxv + ay * dyv + az * dzv;
+}
+
+// One direction of the monotonic limiter, lulesh.cc:1457-1508: delvm/delvp
+// are the neighbour gradients already resolved through the BC switch.
+__device__ __forceinline__ double mono_phi(double self, double delvm, double delvp,
+                                           const Params &P) {
+    const double norm = 1. / (self + 1.e-36);
+    delvm = delvm * norm;
+    delvp = delvp * norm;
+    double phi = .5 * (delvm + delvp);
+    delvm *= P.monoq_limiter_mult;
+    delvp *= P.monoq_limiter_mult;
+    if (delvm < phi) phi = delvm;
+    if (delvp < phi) phi = delvp;
+    if (phi < 0.) phi = 0.;
+    if (phi > P.monoq_max_slope) phi = P.monoq_max_slope;
+    return phi;
+}
+
+// Neighbour gradient through the BC switch (lulesh.cc:1460-1491): *_COMM and
+// "no bit" read the neighbour, *_SYMM the element itself, *_FREE and the
+// switch's default give 0.
+__device__ __forceinline__ double bc_select(int32_t bits, int32_t comm, int32_t symm,
+                                            double nbr, double self) {
+    return (bits == comm || bits == 0) ? nbr : (bits == symm ? self : 0.0);
+}
+
+// qlin / qquad of one element, lulesh.cc:1618-1645
+__device__ __forceinline__ void q_terms(double phixi, double phieta, double phizeta,
+                                        double vdov, const double dv[3], const double dx[3],
+                                        double elemMass, double volo, double vnew,
+                                        const Params &P, double &qlin, double &qquad) {
+    if (vdov > 0.) {
+        qlin = 0.;
+        qquad = 0.;
+        return;
+    }
+    double dxi = dv[0] * dx[0], deta = dv[1] * dx[1], dzeta = dv[2] * dx[2];
+    if (dxi > 0.) dxi = 0.;
+    if (deta > 0.) deta = 0.;
+    if (dzeta > 0.) dzeta = 0.;
+    const double rho = elemMass / (volo * vnew);
+    qlin = -P.qlc_monoq * rho *
+           (dxi * (1. - phixi) + deta * (1. - phieta) + dzeta * (1. - phizeta));
+    qquad = P.qqc_monoq * rho *
+            (dxi * dxi * (1. - phixi * phixi) + deta * deta * (1. - phieta * phieta) +
+             dzeta * dzeta * (1. - phizeta * phizeta));
+}
+
+// CalcPressureForElemsTask for one element, lulesh.cc:1671-1690
+__device__ __forceinline__ double pressure(double &bvc, double &pbvc, double e_old,
+                                           double compression, double vnewc, const Params &P) {
+    const double cls = 2.0 / 3.0;
+    bvc = cls * (compression + 1.0);
+    pbvc = cls;
+    double newval = bvc * e_old;
+    if (fabs_(newval) < P.p_cut || vnewc >= P.eosvmax) newval = 0.0;
+    if (newval < P.pmin) newval = P.pmin;
+    return newval;
+}
+
+// sqrt with the reference's floor, lulesh.cc:1779-1783
+__device__ __forceinline__ double ss_floor_sqrt(double ssc) {
+    return (ssc <= .1111111e-36) ? .3333333e-18 : sqrt_(ssc);
+}
+
+// EvalEOSAllInOneTask for one element (lulesh.cc:1692-1871) followed by the
+// sound speed of CalcSoundSpeedForElemsAndSaveTask (lulesh.cc:1885-1896).
+__device__ __forceinline__ void eos(double e_old, double delvc, double p_old, double q_old,
+                                    double qq_old, double ql_old, double vnewc, const Params &P,
+                                    double &e_out, double &p_out, double &q_out,
+                                    double &ss_out) {
+    const double rho0 = P.refdens;
+    double compression = 1. / vnewc - 1.;
+    const double vchalf = vnewc - delvc * .5;
+    double compHalfStep = 1. / vchalf - 1.;
+    const double work = 0.;
+    if (P.eosvmin != 0. && vnewc <= P.eosvmin) compHalfStep = compression;
+    if (P.eosvmax != 0. && vnewc >= P.eosvmax) {
+        p_old = 0.;
+        compression = 0.;
+        compHalfStep = 0.;
+    }
+    double bvc, pbvc;
+    double e_new = e_old - 0.5 * delvc * (p_old + q_old) + 0.5 * work;
+    if (e_new < P.emin) e_new = P.emin;
+    const double pHalfStep = pressure(bvc, pbvc, e_new, compHalfStep, vnewc, P);
+    double q_new;
+    {
+        const double vhalf = 1. / (1. + compHalfStep);
+        if (delvc > 0.) {
+            q_new = 0.;
+        } else {
+            const double ssc =
+                ss_floor_sqrt((pbvc * e_new + vhalf * vhalf * bvc * pHalfStep) / rho0);
+            q_new = (ssc * ql_old + qq_old);
+        }
+        e_new = e_new + 0.5 * delvc * (3.0 * (p_old + q_old) - 4.0 * (pHalfStep + q_new));
+    }
+    {
+        double nv = e_new + 0.5 * work;
+        if (fabs_(nv) < P.e_cut) nv = 0.0;
+        if (nv < P.emin) nv = P.emin;
+        e_new = nv;
+    }
+    double p_new = pressure(bvc, pbvc, e_new, compression, vnewc, P);
+    {
+        const double sixth = 1.0 / 6.0;
+        double q_tilde;
+        if (delvc > 0.) {
+            q_tilde = 0.;
+        } else {
+            const double ssc =
+                ss_floor_sqrt((pbvc * e_new + vnewc * vnewc * bvc * p_new) / rho0);
+            q_tilde = (ssc * ql_old + qq_old);
+        }
+        e_new = e_new - (7.0 * (p_old + q_old) - 8.0 * (pHalfStep + q_new) + (p_new + q_tilde)) *
+                            delvc * sixth;
+        if (fabs_(e_new) < P.e_cut) e_new = 0.;
+        if (e_new < P.emin) e_new = P.emin;
+    }
+    p_new = pressure(bvc, pbvc, e_new, compression, vnewc, P);
+    if (delvc <= 0.) {
+        const double ssc = ss_floor_sqrt((pbvc * e_new + vnewc * vnewc * bvc * p_new) / rho0);
+        q_new = (ssc * ql_old + qq_old);
+        if (fabs_(q_new) < P.q_cut) q_new = 0.;
+    }
+    ss_out = ss_floor_sqrt((pbvc * e_new + vnewc * vnewc * bvc * p_new) / rho0);
+    e_out = e_new;
+    p_out = p_new;
+    q_out = q_new;
+}
+
+// Courant and hydro constraints of one element, lulesh.cc:1945-1971
+__device__ __forceinline__ void constraints(double ss, double vdov, double arealg,
+                                            const Params &P, double &dtc, double &dth) {
+    if (vdov == 0.0) {
+        dtc = 1.7976931348623157e+308;
+        dth = 1.7976931348623157e+308;
+        return;
+    }
+    const double qqc2 = 64.0 * P.qqc * P.qqc;
+    double dtf = ss * ss;
+    if (vdov < 0.0) dtf += qqc2 * arealg * arealg * vdov * vdov;
+    dtf = sqrt_(dtf);
+    dtc = arealg / dtf;
+    dth = P.dvovmax / (fabs_(vdov) + 1.e-20);
+}
+
+// Region repetition count, lulesh.cc:2159-2169
+__host__ __device__ inline int32_t region_rep(int32_t reg, int32_t numReg, int32_t cost) {
+    if (reg < numReg / 2) return 1;
+    if (reg < numReg - (numReg + 15) / 20) return 1 + cost;
+    return 10 * (1 + cost);
+}
+
+} // namespace lulesh_dev
