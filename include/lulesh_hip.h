@@ -44,6 +44,11 @@ extern "C" {
 /* Arithmetic forms. */
 #define LULESH_PRECISION_FP64 0  /* every kernel in FP64 (the reference's Real_t, lulesh.h:27) */
 #define LULESH_PRECISION_MIXED 1 /* hourglass + monotonic-Q gradients in FP32, the rest FP64 */
+/* Option bits or-ed into the precision argument of lulesh_hip_create. */
+#define LULESH_OPT_NODE_FORCES 0x100 /* keep fx/fy/fz and xdd/ydd/zdd up to date in HBM
+                                        (they are dead between cycles; off by default) */
+#define LULESH_OPT_GENERIC 0x200     /* use the indexed kernels (nodelist + corner CSR)
+                                        even when the mesh is the structured Sedov cube */
 
 /*
  * Plain-pointer view of a Domain (lulesh.h:130-606).  Array lengths:

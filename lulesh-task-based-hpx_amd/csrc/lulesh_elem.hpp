@@ -23,6 +23,16 @@ enum : int32_t {
     ZETA_P = 0x38000, ZETA_P_SYMM = 0x08000, ZETA_P_FREE = 0x10000, ZETA_P_COMM = 0x20000,
 };
 
+// Device-resident time state and flags of one problem (the Domain's time
+// members, lulesh.h:563-572, plus the loop condition of lulesh.cc:2374-2375).
+struct DevState {
+    double time, deltatime, stoptime, dtfixed, deltatimemultlb, deltatimemultub, dtmax;
+    unsigned long long dtc_bits, dth_bits; // constraint minima (bit patterns of positive doubles)
+    int32_t cycle, its, active, error;
+};
+
+constexpr int32_t kVolumeError = -1; // lulesh.h:30
+
 // Run constants (lulesh-init.cc:18-36) passed by value to the kernels.
 struct Params {
     double e_cut, p_cut, q_cut, v_cut, u_cut, hgcoef, monoq_max_slope, monoq_limiter_mult,

@@ -31,17 +31,11 @@
 
 #include "lulesh_elem.hpp"
 #include "lulesh_hip.h"
+#include "lulesh_structured.hpp"
 
 using namespace lulesh_dev;
 
 namespace {
-
-// Device-resident time state and flags of one problem.
-struct DevState {
-    double time, deltatime, stoptime, dtfixed, deltatimemultlb, deltatimemultub, dtmax;
-    unsigned long long dtc_bits, dth_bits; // constraint minima (bit patterns of positive doubles)
-    int32_t cycle, its, active, error;
-};
 
 constexpr int kBlock = 256;
 
@@ -261,9 +255,9 @@ __global__ __launch_bounds__(kBlock) void k_nodes(
 // volume check and UpdateVolumes) and CalcMonotonicQGradientsForElemsTask
 // (lulesh.cc:1277-1418).  Gradients are FP64, or FP32 in the mixed form.
 // ---------------------------------------------------------------------------
-template <typename QG>
+template <typename QG, bool STRUCT>
 __global__ __launch_bounds__(kBlock) void k_elems(
-    int32_t E, const int32_t *__restrict__ nodelist, NodeFields nf,
+    int32_t E, int32_t s, const int32_t *__restrict__ nodelist, NodeFields nf,
     const double *__restrict__ volo, double *__restrict__ v, double *__restrict__ vnew,
     double *__restrict__ delv, double *__restrict__ arealg, double *__restrict__ vdov,
     QG *__restrict__ delv_xi, QG *__restrict__ delv_eta, QG *__restrict__ delv_zeta,
@@ -273,7 +267,17 @@ __global__ __launch_bounds__(kBlock) void k_elems(
     if (i >= E || !st->active) return;
     const double deltaTime = st->deltatime;
     int n[8];
-    load_nodelist(nodelist, i, n);
+    if (STRUCT) {
+        // element (k,j,i) -> its eight nodes, lulesh-init.cc:236-244
+        const int nn = s + 1;
+        const int k = i / (s * s), r = i - k * s * s, j = r / s, ci = r - j * s;
+        const int b = (k * nn + j) * nn + ci;
+        n[0] = b; n[1] = b + 1; n[2] = b + nn + 1; n[3] = b + nn;
+        n[4] = b + nn * nn; n[5] = b + nn * nn + 1; n[6] = b + nn * nn + nn + 1;
+        n[7] = b + nn * nn + nn;
+    } else {
+        load_nodelist(nodelist, i, n);
+    }
     double x[8], y[8], z[8], xd[8], yd[8], zd[8];
     load8(nf.x, n, x);
     load8(nf.y, n, y);
@@ -487,11 +491,19 @@ struct lulesh_ctx {
         int32_t offset, count, rep;
     };
     std::vector<Segment> segs;
+    // structured Sedov cube: fused z-march nodal kernel over a double-buffered
+    // node state (x, y, z, xd, yd, zd); f[] points at the current half
+    bool structured = false;
+    bool store_forces = false; // also write fx.. and xdd.. (Domain fidelity)
+    int zplanes = 12;          // node planes per z-march workgroup
+    double *nodeBuf[2][6] = {};
+    int parity = 0;
     DevState *st = nullptr;
     DevState *hst = nullptr; // pinned mirror
     DevState init{};
     hipGraphExec_t graph = nullptr;
     int graph_cycles = 0;
+    int graph_parity = 0;
     bool timing = false;
     std::vector<KernelTimer> timers;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -574,8 +586,34 @@ int launch(lulesh_ctx *c, int timer_id, F &&fn) {
 
 enum { T_TIME = 0, T_FORCE, T_NODES, T_ELEMS, T_MATERIAL, T_COUNT };
 
+void set_node_parity(lulesh_ctx *c, int parity) {
+    c->parity = parity;
+    for (int k = 0; k < 6; ++k) c->f[LULESH_F_X + k] = c->nodeBuf[parity][k];
+}
+
 template <typename R>
 int enqueue_nodal(lulesh_ctx *c) {
+    if (c->structured) {
+        const int cur = c->parity, nxt = cur ^ 1;
+        double *const *a = c->nodeBuf[cur];
+        double *const *b = c->nodeBuf[nxt];
+        const NodeIn in{a[0], a[1], a[2], a[3], a[4], a[5]};
+        const NodeOut out{b[0], b[1], b[2], b[3], b[4], b[5]};
+        const NodeForceOut fo{c->f[LULESH_F_FX], c->f[LULESH_F_FY], c->f[LULESH_F_FZ],
+                              c->f[LULESH_F_XDD], c->f[LULESH_F_YDD], c->f[LULESH_F_ZDD]};
+        const int n = c->s + 1;
+        const int tiles = (n + kTile - 1) / kTile;
+        const dim3 grid(unsigned(tiles * tiles), unsigned((n + c->zplanes - 1) / c->zplanes));
+        int rc = launch(c, T_FORCE, [&] {
+            auto kern = c->store_forces ? k_nodal_zmarch<R, true> : k_nodal_zmarch<R, false>;
+            kern<<<grid, kZmarchThreads, 0, c->stream>>>(
+                c->s, c->zplanes, in, out, fo, c->f[LULESH_F_P], c->f[LULESH_F_Q],
+                c->f[LULESH_F_VOLO], c->f[LULESH_F_V], c->f[LULESH_F_SS], c->f[LULESH_F_ELEMMASS],
+                c->f[LULESH_F_NODALMASS], c->hgcoef, c->P.u_cut, c->st);
+        });
+        set_node_parity(c, nxt);
+        return rc;
+    }
     const NodeFields nf{c->f[LULESH_F_X], c->f[LULESH_F_Y], c->f[LULESH_F_Z],
                         c->f[LULESH_F_XD], c->f[LULESH_F_YD], c->f[LULESH_F_ZD]};
     int rc = launch(c, T_FORCE, [&] {
@@ -605,8 +643,9 @@ int enqueue_elements(lulesh_ctx *c) {
         g[k] = sizeof(R) == 8 ? reinterpret_cast<R *>(c->f[LULESH_F_DELV_XI + k])
                               : reinterpret_cast<R *>(c->g32[k]);
     return launch(c, T_ELEMS, [&] {
-        k_elems<R><<<grid_for(c->E, kBlock), kBlock, 0, c->stream>>>(
-            c->E, c->nodelist, nf, c->f[LULESH_F_VOLO], c->f[LULESH_F_V], c->f[LULESH_F_VNEW],
+        auto kern = c->structured ? k_elems<R, true> : k_elems<R, false>;
+        kern<<<grid_for(c->E, kBlock), kBlock, 0, c->stream>>>(
+            c->E, c->s, c->nodelist, nf, c->f[LULESH_F_VOLO], c->f[LULESH_F_V], c->f[LULESH_F_VNEW],
             c->f[LULESH_F_DELV], c->f[LULESH_F_AREALG], c->f[LULESH_F_VDOV], g[0], g[1], g[2],
             g[3], g[4], g[5], c->P.v_cut, c->P.eosvmin, c->P.eosvmax, c->st);
     });
@@ -654,7 +693,10 @@ int enqueue_cycle(lulesh_ctx *c) {
 }
 
 int build_graph(lulesh_ctx *c, int cycles) {
-    if (c->graph && c->graph_cycles == cycles) return 0;
+    // an even number of cycles returns the node double buffer to the parity
+    // the graph was captured with
+    cycles += cycles & 1;
+    if (c->graph && c->graph_cycles == cycles && c->graph_parity == c->parity) return 0;
     if (c->graph) {
         (void)hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
@@ -673,6 +715,7 @@ int build_graph(lulesh_ctx *c, int cycles) {
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return hip_fail(c, e, "hipGraphInstantiate");
     c->graph_cycles = cycles;
+    c->graph_parity = c->parity;
     return 0;
 }
 
@@ -684,9 +727,61 @@ int read_state(lulesh_ctx *c) {
 
 } // namespace
 
+namespace {
+
+// True when the Domain is the structured Sedov cube of the reference set-up
+// (lulesh-init.cc:202-310, 448-468): canonical connectivity, corner lists in
+// ascending element order, and the three symmetry sets are the x = 0, y = 0
+// and z = 0 node planes.  Only then may the index-arithmetic kernels run.
+bool is_structured_cube(const lulesh_domain_view *v) {
+    const int64_t s = v->sizeX, n = s + 1, E = v->numElem, N = v->numNode;
+    std::vector<int32_t> count(size_t(N) + 1, 0);
+    for (int64_t e = 0; e < E; ++e) {
+        const int64_t k = e / (s * s), j = (e / s) % s, i = e % s;
+        const int64_t b = (k * n + j) * n + i;
+        const int64_t nd[8] = {b, b + 1, b + n + 1, b + n, b + n * n, b + n * n + 1,
+                               b + n * n + n + 1, b + n * n + n};
+        for (int a = 0; a < 8; ++a) {
+            if (v->nodelist[8 * e + a] != nd[a]) return false;
+            ++count[size_t(nd[a]) + 1];
+        }
+    }
+    for (int64_t g = 0; g < N; ++g) {
+        count[size_t(g) + 1] += count[size_t(g)];
+        if (v->nodeElemStart[g] != count[size_t(g)]) return false;
+    }
+    if (v->nodeElemStart[N] != 8 * E) return false;
+    std::vector<int32_t> fill(count.begin(), count.end() - 1);
+    for (int64_t c = 0; c < 8 * E; ++c) {
+        const int32_t g = v->nodelist[c];
+        if (v->nodeElemCornerList[fill[size_t(g)]++] != int32_t(c)) return false;
+    }
+    if (!v->symmX || !v->symmY || !v->symmZ) return false;
+    const int64_t P = n * n;
+    for (int axis = 0; axis < 3; ++axis) {
+        const int32_t *set = axis == 0 ? v->symmX : axis == 1 ? v->symmY : v->symmZ;
+        std::vector<int32_t> got(set, set + P), want;
+        want.reserve(size_t(P));
+        for (int64_t a = 0; a < n; ++a)
+            for (int64_t b = 0; b < n; ++b)
+                want.push_back(int32_t(axis == 0 ? (a * n + b) * n        // I == 0
+                                      : axis == 1 ? (a * n) * n + b       // J == 0
+                                                  : a * n + b));          // K == 0
+        std::sort(got.begin(), got.end());
+        std::sort(want.begin(), want.end());
+        if (got != want) return false;
+    }
+    return true;
+}
+
+} // namespace
+
 extern "C" {
 
-int lulesh_hip_create(const lulesh_domain_view *v, int precision, lulesh_ctx **out) {
+int lulesh_hip_create(const lulesh_domain_view *v, int precision_opts, lulesh_ctx **out) {
+    const int precision = precision_opts & 0xff;
+    if (precision_opts & ~(0xff | LULESH_OPT_NODE_FORCES | LULESH_OPT_GENERIC))
+        return LULESH_INVALID_ARGUMENT;
     if (!v || !out || v->sizeX < 1 || v->numElem != v->sizeX * v->sizeX * v->sizeX ||
         v->numNode != (v->sizeX + 1) * (v->sizeX + 1) * (v->sizeX + 1) || v->numReg < 1 ||
         (precision != LULESH_PRECISION_FP64 && precision != LULESH_PRECISION_MIXED) ||
@@ -698,6 +793,8 @@ int lulesh_hip_create(const lulesh_domain_view *v, int precision, lulesh_ctx **o
     *out = nullptr;
     auto *c = new lulesh_ctx();
     c->precision = precision;
+    c->store_forces = (precision_opts & LULESH_OPT_NODE_FORCES) != 0;
+    c->structured = !(precision_opts & LULESH_OPT_GENERIC) && is_structured_cube(v);
     c->s = v->sizeX;
     c->E = v->numElem;
     c->N = v->numNode;
@@ -728,8 +825,14 @@ int lulesh_hip_create(const lulesh_domain_view *v, int precision, lulesh_ctx **o
         if (hipMemset(c->f[id], 0, n * sizeof(double)) != hipSuccess) return fail(LULESH_HIP_ERROR);
         if ((rc = upload(c, c->f[id], (const double *)view_field(v, id), n))) return fail(rc);
     }
-    if ((rc = dalloc(c, &c->nodelist, 8 * E)) || (rc = upload(c, c->nodelist, v->nodelist, 8 * E)))
+    for (int k = 0; k < 6; ++k) c->nodeBuf[0][k] = c->f[LULESH_F_X + k];
+    if (c->structured) {
+        for (int k = 0; k < 6; ++k)
+            if ((rc = dalloc(c, &c->nodeBuf[1][k], N))) return fail(rc);
+    } else if ((rc = dalloc(c, &c->nodelist, 8 * E)) ||
+               (rc = upload(c, c->nodelist, v->nodelist, 8 * E))) {
         return fail(rc);
+    }
     int32_t **conn[7] = {&c->lxim, &c->lxip, &c->letam, &c->letap, &c->lzetam, &c->lzetap,
                          &c->elemBC};
     const int32_t *hconn[7] = {v->lxim, v->lxip, v->letam, v->letap, v->lzetam, v->lzetap,
@@ -740,8 +843,8 @@ int lulesh_hip_create(const lulesh_domain_view *v, int precision, lulesh_ctx **o
             return fail(rc);
     }
     // Corner CSR with corner ids remapped to the corner-major layout of the
-    // force arrays: corner 8*e+k lives at k*E+e.
-    {
+    // force arrays: corner 8*e+k lives at k*E+e (indexed path only).
+    if (!c->structured) {
         std::vector<int32_t> cmap(8 * E);
         for (size_t i = 0; i < 8 * E; ++i) {
             const int32_t cl = v->nodeElemCornerList[i];
@@ -754,7 +857,7 @@ int lulesh_hip_create(const lulesh_domain_view *v, int precision, lulesh_ctx **o
             return fail(rc);
     }
     // Symmetry-plane node sets as a per-node mask (bit0 x, bit1 y, bit2 z).
-    {
+    if (!c->structured) {
         std::vector<uint8_t> mask(N, 0);
         const size_t P = size_t(c->s + 1) * (c->s + 1);
         const int32_t *sets[3] = {v->symmX, v->symmY, v->symmZ};
@@ -788,7 +891,7 @@ int lulesh_hip_create(const lulesh_domain_view *v, int precision, lulesh_ctx **o
     }
     // Corner force arrays (allocated once; the reference mallocs them every
     // cycle, lulesh.cc:2017-2022).
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 3 && !c->structured; ++k) {
         if ((rc = dalloc(c, &c->cs[k], 8 * E))) return fail(rc);
         if (precision == LULESH_PRECISION_MIXED) {
             float *p = nullptr;
@@ -813,6 +916,14 @@ int lulesh_hip_create(const lulesh_domain_view *v, int precision, lulesh_ctx **o
             (void)hipStreamSynchronize(c->stream);
             (void)hipFree(tmp);
         }
+    }
+    if (c->structured) {
+        // z extent of a z-march workgroup: enough workgroups to fill the chip
+        // (about 4 per CU) while keeping the one-plane halo small.
+        const int n = c->s + 1, tiles = (n + kTile - 1) / kTile;
+        int zp = n;
+        while (zp > 4 && int64_t(tiles) * tiles * ((n + zp - 1) / zp) < 1024) zp = (zp + 1) / 2;
+        c->zplanes = std::max(zp, 1);
     }
     if ((rc = dalloc(c, &c->st, 1))) return fail(rc);
     if (hipHostMalloc(reinterpret_cast<void **>(&c->hst), sizeof(DevState)) != hipSuccess)

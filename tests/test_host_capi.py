@@ -1,0 +1,114 @@
+"""CPU checks of the product's host side and C-ABI surface (no GPU needed):
+the library loads and exports every declared entry point, the host Domain
+is bit-identical to the oracle's set-up, the host TimeIncrement and report
+match the oracle, and argument errors are rejected before any device call.
+"""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle_lib import OracleDomain
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INT_FIELDS = ["nodelist", "lxim", "lxip", "letam", "letap", "lzetam", "lzetap", "elemBC",
+              "regNumList", "symmX", "symmY", "symmZ", "nodeElemStart", "nodeElemCornerList"]
+
+
+def declared_functions():
+    names = set()
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(re.findall(r"\b(lulesh_\w+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol(pyl):
+    L = pyl.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("s,nr,b,c", [(1, 11, 1, 1), (2, 11, 1, 1), (5, 11, 1, 1), (9, 16, 1, 2),
+                                      (12, 21, 1, 1), (7, 5, 2, 0), (6, 1, 1, 1), (30, 11, 1, 1)])
+def test_host_domain_matches_oracle(pyl, s, nr, b, c):
+    dom = pyl.Domain(s, nr, b, c)
+    ora = OracleDomain(s, nr, b, c)
+    for f in pyl.FIELDS:
+        np.testing.assert_array_equal(dom.field(f), ora.f64(f), err_msg=f)
+    for f in INT_FIELDS:
+        np.testing.assert_array_equal(dom.field(f), ora.i32(f), err_msg=f)
+    for r in range(nr):
+        np.testing.assert_array_equal(dom.region(r), ora.region(r))
+        assert pyl.lib().lulesh_region_rep(r, nr, c) == ora.rep(r)
+    v = dom.view
+    for k in ("deltatime", "time", "stoptime", "dtcourant", "dthydro", "dtmax", "dtfixed",
+              "hgcoef", "qqc", "eosvmin", "eosvmax", "emin", "qlc_monoq", "qqc_monoq"):
+        assert getattr(v, k) == ora.scalar(k), k
+
+
+def test_host_time_increment_matches_oracle(pyl):
+    rng = np.random.default_rng(0)
+    dom = pyl.Domain(2)
+    for trial in range(300):
+        ora = OracleDomain(2)
+        st = pyl.TimeState()
+        st.time = float(rng.uniform(0, 1.2e-2))
+        st.deltatime = float(10 ** rng.uniform(-6, -2))
+        st.dtcourant = float(10 ** rng.uniform(-7, -1)) if rng.random() < 0.8 else 1e20
+        st.dthydro = float(10 ** rng.uniform(-7, -1)) if rng.random() < 0.8 else 1e20
+        st.cycle = int(rng.integers(0, 3))
+        dom.set_time_state(st)
+        for k in ("time", "deltatime", "dtcourant", "dthydro"):
+            ora.set_scalar(k, getattr(st, k))
+        ora.cycle = st.cycle
+        dom.time_increment()
+        ora.time_increment()
+        got = dom.time_state()
+        assert got.deltatime == ora.scalar("deltatime"), trial
+        assert got.time == ora.scalar("time"), trial
+        assert got.cycle == ora.cycle
+
+
+def test_host_report_matches_oracle(pyl):
+    rng = np.random.default_rng(1)
+    s = 9
+    dom = pyl.Domain(s)
+    ora = OracleDomain(s)
+    e = rng.standard_normal(s ** 3) * 1e3
+    e[rng.integers(0, s ** 3, 50)] = 0.0
+    dom.field("e")[:] = e
+    ora.f64("e")[:] = e
+    np.testing.assert_array_equal(dom.report(), ora.report())
+
+
+def test_create_rejects_bad_views_without_device(pyl):
+    L = pyl.lib()
+    h = ctypes.c_void_p()
+    assert L.lulesh_hip_create(None, 0, ctypes.byref(h)) == pyl.INVALID_ARGUMENT
+    dom = pyl.Domain(3)
+    bad = pyl.DomainView.from_buffer_copy(dom.view)
+    bad.numElem = 26
+    assert L.lulesh_hip_create(ctypes.byref(bad), 0, ctypes.byref(h)) == pyl.INVALID_ARGUMENT
+    assert L.lulesh_hip_create(ctypes.byref(dom.view), 7, ctypes.byref(h)) == pyl.INVALID_ARGUMENT
+    assert L.lulesh_domain_create(0, 11, 1, 1) is None
+    assert L.lulesh_hip_leapfrog(None, 1e-6, None, None) == pyl.INVALID_ARGUMENT
+    assert L.lulesh_hip_run(None, 10, None) == pyl.INVALID_ARGUMENT
+
+
+def test_driver_flag_errors():
+    exe = os.path.join(ROOT, "lulesh-task-based-hpx_amd", "lulesh-amd")
+    if not os.path.exists(exe):
+        pytest.skip("driver not built")
+    r = subprocess.run([exe, "--bogus"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 255 and "usage" in r.stderr
+    r = subprocess.run([exe, "--precision", "fp8"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 255
