@@ -40,24 +40,22 @@ def model_bytes_per_zone_cycle(s):
 
 
 def kernel_bytes(name, E, N, mixed=False):
-    """Algorithmic bytes moved by one launch of each kernel over the mesh
-    (every array element counted once per launch; node gathers once per
-    unique node)."""
-    hg = 4 if mixed else 8
+    """Algorithmic HBM bytes of one launch of each kernel over the mesh
+    (structured path): every array element counted once per launch, node
+    arrays once per node."""
     gr = 4 if mixed else 8
-    if name == "force":
-        # nodelist, 6 node fields, p q volo v ss elemMass -> 24 stress + 24 hourglass corners
-        return E * (32 + 48 + 24 * 8 + 24 * hg) + N * 48
-    if name == "nodes":
-        # CSR start, corner map, 6 corner arrays, mass, bc mask, 6 node fields in,
-        # 12 node fields out (x..zd, fx..fz, xdd..zdd)
-        return E * (32 + 24 * 8 + 24 * hg) + N * (4 + 8 + 1 + 48 + 96)
+    if name == "nodal":
+        # z-march: node x..zd + nodalMass in, x..zd out; p q volo v ss elemMass in
+        return E * 48 + N * (48 + 8 + 48)
     if name == "elements":
-        return E * (32 + 16 + 40 + 6 * gr) + N * 48
+        # node x..zd in; volo v in; v vnew delv arealg vdov out; 6 gradients out
+        return E * (16 + 40 + 6 * gr) + N * 48
     if name == "material":
+        # list, elemBC, 6 neighbour indices, 6 gradients, vdov elemMass volo vnew e
+        # delv p q arealg in; qq ql p e q ss out
         return E * (4 + 4 + 24 + 6 * gr + 72 + 48)
     if name == "time_increment":
-        return 64
+        return 128
     return 0
 
 
@@ -79,7 +77,13 @@ def cpu_baseline(s, nr, seconds, threads):
     return s ** 3 * cycles / el, cycles, el
 
 
-def timed_run(pyl, dev, dom, warmup, steps):
+def timed_run(pyl, dev, dom, warmup, steps, tune_cycles):
+    """Autotune (from a snapshot that is restored), warm up, then time
+    `steps` cycles with the fields resident in HBM."""
+    tuning = None
+    if tune_cycles:
+        t, _ = dev.autotune(tune_cycles)
+        tuning = t.as_dict()
     ts = dom.time_state()
     if warmup:
         rc = dev.run(warmup, ts)
@@ -88,19 +92,49 @@ def timed_run(pyl, dev, dom, warmup, steps):
     rc = dev.run(ts.cycle + steps, ts)
     t1 = time.perf_counter()
     assert rc == 0, rc
-    return t1 - t0, ts
+    return t1 - t0, ts, tuning
+
+
+def batch_bench(pyl, nprob, its, seed, tune_cycles, precision):
+    """The ensemble: nprob Sedov problems with seeded sizes in [20, 80],
+    region counts in {11, 16, 21} and energy factors in [0.5, 2), each run
+    for `its` cycles (or to its stop time) in one context."""
+    sizes, regions, factors = pyl.batch_params(seed, nprob)
+    doms = [pyl.make_domain(int(s), int(r), float(f)) for s, r, f in zip(sizes, regions, factors)]
+    b = pyl.Batch(doms, precision)
+    tuning = None
+    if tune_cycles:
+        t, _ = b.autotune(tune_cycles)
+        tuning = t.as_dict()
+    t0 = time.perf_counter()
+    rc, states = b.run(its)
+    el = time.perf_counter() - t0
+    assert rc == 0, rc
+    zc = sum(int(s) ** 3 * st.cycle for s, st in zip(sizes, states))
+    out = {"value": zc / el, "unit": "zone-cycles/s", "problems": nprob,
+           "elements": int(sum(int(s) ** 3 for s in sizes)), "zone_cycles": zc,
+           "elapsed_s": el, "device_ms": b.last_run_ms(),
+           "cycles_min": min(st.cycle for st in states),
+           "cycles_max": max(st.cycle for st in states), "seed": seed, "tuning": tuning,
+           "dtype": "f64" if precision == pyl.FP64 else "mixed"}
+    b.close()
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--s", type=int, default=150)
     ap.add_argument("--regions", type=int, default=11)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--batch-its", type=int, default=1000)
+    ap.add_argument("--tune-cycles", type=int, default=6)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
@@ -116,7 +150,7 @@ def main():
     print(f"[bench] building Domain s={s}", file=sys.stderr, flush=True)
     dom = pyl.Domain(s, nr)
     dev = pyl.Device(dom, pyl.FP64)
-    elapsed, ts = timed_run(pyl, dev, dom, args.warmup, args.steps)
+    elapsed, ts, tuning = timed_run(pyl, dev, dom, args.warmup, args.steps, args.tune_cycles)
     cycles = args.steps
     zcps = E * cycles / elapsed
     print(f"[bench] fp64: {cycles} cycles in {elapsed:.3f}s -> {zcps:.4e} zc/s",
@@ -155,7 +189,7 @@ def main():
     if not args.no_mixed:
         domm = pyl.Domain(s, nr)
         devm = pyl.Device(domm, pyl.MIXED)
-        el_m, tsm = timed_run(pyl, devm, domm, args.warmup, args.steps)
+        el_m, tsm, _ = timed_run(pyl, devm, domm, args.warmup, args.steps, args.tune_cycles)
         devm.download_view()
         e0m = float(domm.field("e")[0])
         mixed = {"value": E * args.steps / el_m, "ms_per_step": 1e3 * el_m / args.steps,
@@ -163,6 +197,13 @@ def main():
                  "cycles": tsm.cycle}
         devm.close()
         domm.close()
+
+    # ---- 64-problem batch -------------------------------------------------
+    batch = None
+    if not args.no_batch:
+        print(f"[bench] batch of {args.batch}", file=sys.stderr, flush=True)
+        batch = batch_bench(pyl, args.batch, args.batch_its, 0, args.tune_cycles, pyl.FP64)
+        print(f"[bench] batch: {batch['value']:.4e} zc/s", file=sys.stderr, flush=True)
 
     # ---- CPU baseline ----------------------------------------------------
     cpu = None
@@ -200,7 +241,9 @@ def main():
         "ms_per_cycle_kernel_sum": round(ms_cycle_events, 4),
         "origin_energy": e0_fp64,
         "symmetry_max_abs_diff": float(rep64[1]),
+        "tuning": tuning,
         "mixed": mixed,
+        "batch": batch,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (zcps / cpu["value"]) if cpu else None,
     }

@@ -46,6 +46,18 @@ void lulesh_verify_report(lulesh_domain *d, double out[4]);
 /* Region repetition count of the EOS (lulesh.cc:2159-2169). */
 int32_t lulesh_region_rep(int32_t reg, int32_t numReg, int32_t cost);
 
+/* Scale the energy deposited in element 0 by `factor` and recompute the
+ * initial time step from it, as the constructor does for the unscaled
+ * deposit (lulesh-init.cc:172-181).  For ensembles of Sedov problems. */
+int lulesh_domain_scale_energy(lulesh_domain *d, double factor);
+
+/* Parameters of a seeded ensemble (no counterpart in the reference): problem
+ * sizes uniform in [smin, smax], region counts from {11, 16, 21}, energy
+ * factors uniform in [0.5, 2), all drawn from one std::mt19937_64 stream
+ * (the reference's libc rand() stream is left untouched). */
+int lulesh_batch_params(uint64_t seed, int32_t nprob, int32_t smin, int32_t smax,
+                        int32_t *sizes, int32_t *regions, double *factors);
+
 #ifdef __cplusplus
 }
 #endif

@@ -159,6 +159,45 @@ int lulesh_hip_kernel_times(lulesh_ctx *ctx, const char **names, double *avg_ms,
 
 /* Device memory held by the context, in bytes. */
 int64_t lulesh_hip_device_bytes(const lulesh_ctx *ctx);
+/* 1 if the structured (index-arithmetic, fused z-march) kernels are in use. */
+int32_t lulesh_hip_is_structured(const lulesh_ctx *ctx);
+/* Device milliseconds of the last lulesh_hip_run / lulesh_hip_batch_run
+ * (HIP events around the replayed graphs, polling included). */
+double lulesh_hip_last_run_ms(const lulesh_ctx *ctx);
+
+/* ---------------------------------------------------------------------- */
+/* Launch tuning.  No counterpart in the reference, whose knob is the HPX
+ * task size (--task-size / --elems-per-task, lulesh.cc:2267-2335). */
+typedef struct lulesh_tuning {
+    int32_t zplanes;          /* node planes per z-march workgroup (0 = sized to fill the GPU) */
+    int32_t block_elem;       /* threads per block of the kinematics kernel: 64, 128, 256 */
+    int32_t block_material;   /* threads per block of the material/EOS kernel: 64, 128, 256 */
+    int32_t cycles_per_graph; /* cycles captured per HIP graph (0 = by problem size) */
+} lulesh_tuning;
+
+int lulesh_hip_set_tuning(lulesh_ctx *ctx, const lulesh_tuning *tuning);
+int lulesh_hip_get_tuning(const lulesh_ctx *ctx, lulesh_tuning *tuning);
+/* Autotuner: times `cycles` cycles for each candidate launch configuration
+ * (coordinate descent over zplanes, block_elem, block_material,
+ * cycles_per_graph) starting from a snapshot of the current state, restores
+ * the state, and keeps the fastest.  Returns it and its ms per cycle. */
+int lulesh_hip_autotune(lulesh_ctx *ctx, int32_t cycles, lulesh_tuning *best,
+                        double *best_ms_per_cycle);
+
+/* ---------------------------------------------------------------------- */
+/* Batched ensemble: many independent Sedov Domains in one context, advanced
+ * together (one launch per kernel covers every problem) while each keeps its
+ * own time step, time and cycle -- exactly as if run alone.  No counterpart
+ * in the reference, which runs one Domain per process.  All problems must
+ * be structured cubes with the same run constants. */
+int lulesh_hip_batch_create(const lulesh_domain_view *const *views, int32_t nprob,
+                            int precision, lulesh_ctx **out);
+/* Run every problem until its stop time or max_cycles; states[nprob] receive
+ * the final time states (status = error code per problem). */
+int lulesh_hip_batch_run(lulesh_ctx *ctx, int32_t max_cycles, lulesh_time_state *states);
+int lulesh_hip_batch_download(lulesh_ctx *ctx, int32_t prob, int field_id, void *dst,
+                              size_t count);
+int32_t lulesh_hip_num_problems(const lulesh_ctx *ctx);
 
 const char *lulesh_hip_last_error(const lulesh_ctx *ctx);
 void lulesh_hip_destroy(lulesh_ctx *ctx);

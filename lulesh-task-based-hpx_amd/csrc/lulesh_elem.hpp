@@ -25,10 +25,21 @@ enum : int32_t {
 
 // Device-resident time state and flags of one problem (the Domain's time
 // members, lulesh.h:563-572, plus the loop condition of lulesh.cc:2374-2375).
+// `cur` selects the half of the node double buffer that holds the state at
+// the start of the cycle; the nodal phase writes the other half, and the
+// next cycle adopts it only if this cycle ran (`active`), so cycles skipped
+// after the stop condition leave the buffers consistent.
 struct DevState {
     double time, deltatime, stoptime, dtfixed, deltatimemultlb, deltatimemultub, dtmax;
     unsigned long long dtc_bits, dth_bits; // constraint minima (bit patterns of positive doubles)
     int32_t cycle, its, active, error;
+    int32_t cur, pad;
+};
+
+// Both halves of the node double buffer: b[h][0..5] = x, y, z, xd, yd, zd.
+// The indexed path has a single buffer and passes it twice.
+struct NodeBufs {
+    double *b[2][6];
 };
 
 constexpr int32_t kVolumeError = -1; // lulesh.h:30

@@ -2,31 +2,38 @@
 // its C-ABI (include/lulesh_hip.h).
 //
 // One cycle of the reference (TimeIncrement, lulesh.cc:182-229, then
-// LagrangeLeapFrogWithTasks, lulesh.cc:1977-2240) becomes a short sequence of
-// stream-ordered kernels over device-resident SoA fields:
+// LagrangeLeapFrogWithTasks, lulesh.cc:1977-2240) is four stream-ordered
+// kernel launches over device-resident SoA fields:
 //
-//   k_time_increment  one thread: new dt from the previous cycle's minima
-//   k_force           per element: stress (IntegrateStress) and FB hourglass
-//                     corner forces, written corner-major (coalesced)
-//   k_nodes           per node: corner->node gather in the reference's
-//                     ascending-element order (no atomics), acceleration,
-//                     symmetry BCs, velocity and position
+//   k_time_increment  one thread per problem: new dt from the previous
+//                     cycle's minima, loop condition, minima reset
+//   k_nodal_zmarch    the whole nodal phase fused (structured Sedov cube):
+//                     stress + FB hourglass corner forces staged in LDS and
+//                     gathered to nodes in the reference's corner order, then
+//                     acceleration, symmetry BCs, velocity and position
+//                     (lulesh_structured.hpp).  Meshes with any other
+//                     connectivity use k_force + k_nodes (indexed gathers
+//                     over nodelist and the corner CSR, corner forces in HBM).
 //   k_elems           per element: kinematics, volume update, Q gradients
-//   k_material        per element of a region list: monotonic Q, EOS x rep,
-//                     sound speed, and the dt minima (wave shuffles + one
-//                     64-bit atomic min per block on the bit pattern of the
-//                     positive double, which is exact and order-free)
+//   k_material        per element of the region lists, grouped by EOS
+//                     repetition count: monotonic Q, EOS x rep, sound speed,
+//                     and the element's dt constraints folded into a block
+//                     minimum and one 64-bit atomic min per block on the bit
+//                     pattern of the (positive) double -- exact, order-free.
 //
-// Cycles are captured in a HIP graph and replayed; every kernel reads an
-// `active` flag written by k_time_increment, so cycles past the stop time or
-// cycle limit are no-ops and the host only polls the time state between
-// graph launches.
+// A context holds one problem or a batch of independent problems stored back
+// to back; every launch covers all problems through block -> problem tables,
+// and each problem keeps its own time state (DevState).  Cycles are captured
+// in a HIP graph and replayed; kernels of a problem whose loop condition is
+// false exit at once, so the host only polls the time states between graph
+// launches.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "lulesh_elem.hpp"
@@ -37,7 +44,7 @@ using namespace lulesh_dev;
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 256; // default and maximum block of the element kernels
 
 __device__ __forceinline__ unsigned long long dbits(double v) {
     return (unsigned long long)__double_as_longlong(v);
@@ -54,11 +61,15 @@ inline unsigned long long host_dbits(double v) {
 }
 
 // ---------------------------------------------------------------------------
-// TimeIncrement on the device, lulesh.cc:182-229.  The loop condition of
-// lulesh.cc:2374-2375 decides whether this cycle runs at all.
+// TimeIncrement on the device, lulesh.cc:182-229, one thread per problem.
+// The loop condition of lulesh.cc:2374-2375 decides whether the cycle runs.
 // ---------------------------------------------------------------------------
-__global__ void k_time_increment(DevState *st) {
+__global__ void k_time_increment(DevState *states, int32_t nprob) {
+    const int pi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= nprob) return;
+    DevState *st = states + pi;
     DevState s = *st;
+    if (s.active) st->cur = s.cur ^ 1; // adopt the node state the last cycle wrote
     const bool run = !s.error && (s.time < s.stoptime) && (s.cycle < s.its);
     st->active = run ? 1 : 0;
     if (!run) return;
@@ -93,8 +104,10 @@ __global__ void k_time_increment(DevState *st) {
     st->dth_bits = dbits(1.0e+20);
 }
 
-// A cycle driven by a host-side TimeIncrement: take its dt, reset minima.
-__global__ void k_begin_cycle(DevState *st, double deltatime, int reset_minima) {
+// A cycle (or a nodal phase) driven by a host-side TimeIncrement: adopt the
+// previous cycle's node state if asked, take the host's dt, reset minima.
+__global__ void k_begin_cycle(DevState *st, double deltatime, int reset_minima, int adopt) {
+    if (adopt && st->active) st->cur ^= 1;
     st->deltatime = deltatime;
     st->active = 1;
     st->error = 0;
@@ -122,10 +135,9 @@ struct NodeFields {
 };
 
 // ---------------------------------------------------------------------------
-// Element corner forces: InitIntegrateStressForElemsTask (lulesh.cc:787-836)
-// and CalcHourglassForElemsTask (lulesh.cc:864-1113).  Stress corners are
-// FP64; hourglass corners are FP64 or, in the mixed form, FP32 computed from
-// node-0-relative coordinates.
+// Indexed path (any connectivity, one problem).  Element corner forces:
+// InitIntegrateStressForElemsTask (lulesh.cc:787-836) and
+// CalcHourglassForElemsTask (lulesh.cc:864-1113), written corner-major.
 // ---------------------------------------------------------------------------
 template <typename HG>
 __global__ __launch_bounds__(kBlock) void k_force(
@@ -143,10 +155,9 @@ __global__ __launch_bounds__(kBlock) void k_force(
     load8(nf.x, n, x);
     load8(nf.y, n, y);
     load8(nf.z, n, z);
-    // stress part
     {
         const double det = shape_derivatives<false>(x, y, z, nullptr);
-        if (det == 0) st->error = LULESH_VOLUME_ERROR; // lulesh.cc:819-822
+        if (det == 0) st->error = kVolumeError; // lulesh.cc:819-822
         double B[3][8];
         node_normals(B, x, y, z);
         const double sig = -p[i] - q[i];
@@ -158,9 +169,8 @@ __global__ __launch_bounds__(kBlock) void k_force(
             csz[o] = -(sig * B[2][k]);
         }
     }
-    // hourglass part
     const double determ = volo[i] * v[i];
-    if (determ <= 0.0) st->error = LULESH_VOLUME_ERROR; // lulesh.cc:906-910
+    if (determ <= 0.0) st->error = kVolumeError; // lulesh.cc:906-910
     double xd[8], yd[8], zd[8];
     load8(nf.xd, n, xd);
     load8(nf.yd, n, yd);
@@ -194,13 +204,10 @@ __global__ __launch_bounds__(kBlock) void k_force(
     }
 }
 
-// ---------------------------------------------------------------------------
-// Nodes: combineVolumeForcesTaskFunc (lulesh.cc:838-862) in the reference's
-// corner order with stress and hourglass terms interleaved,
-// CalcAccelerationForNodesTask (1115-1123), the symmetry-plane BCs
-// (566-588, as a per-node mask), CalcVelocityAndPositionForNodesTask
-// (1125-1158).
-// ---------------------------------------------------------------------------
+// Indexed path, nodes: combineVolumeForcesTaskFunc (lulesh.cc:838-862) in the
+// reference's corner order with stress and hourglass interleaved,
+// CalcAccelerationForNodesTask (1115-1123), symmetry BCs (566-588, as a
+// per-node mask), CalcVelocityAndPositionForNodesTask (1125-1158).
 template <typename HG>
 __global__ __launch_bounds__(kBlock) void k_nodes(
     int32_t N, const int32_t *__restrict__ start, const int32_t *__restrict__ cornerMap,
@@ -254,44 +261,55 @@ __global__ __launch_bounds__(kBlock) void k_nodes(
 // Elements: CalcKinematicsForElemsTask (lulesh.cc:1169-1275, including the
 // volume check and UpdateVolumes) and CalcMonotonicQGradientsForElemsTask
 // (lulesh.cc:1277-1418).  Gradients are FP64, or FP32 in the mixed form.
+// STRUCT: node indices from the element's (k,j,i); else from nodelist.
+// The end-of-step node state is the half written by the nodal phase.
 // ---------------------------------------------------------------------------
+struct ElemKin {
+    const double *volo;
+    double *v, *vnew, *delv, *arealg, *vdov;
+};
+
 template <typename QG, bool STRUCT>
 __global__ __launch_bounds__(kBlock) void k_elems(
-    int32_t E, int32_t s, const int32_t *__restrict__ nodelist, NodeFields nf,
-    const double *__restrict__ volo, double *__restrict__ v, double *__restrict__ vnew,
-    double *__restrict__ delv, double *__restrict__ arealg, double *__restrict__ vdov,
-    QG *__restrict__ delv_xi, QG *__restrict__ delv_eta, QG *__restrict__ delv_zeta,
-    QG *__restrict__ delx_xi, QG *__restrict__ delx_eta, QG *__restrict__ delx_zeta,
-    double v_cut, double eosvmin, double eosvmax, DevState *st) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= E || !st->active) return;
+    const ProbDesc *__restrict__ pds, const int32_t *__restrict__ blockProb,
+    const int32_t *__restrict__ nodelist, NodeBufs nb, ElemKin f0, QG *__restrict__ dvxi,
+    QG *__restrict__ dveta, QG *__restrict__ dvzeta, QG *__restrict__ dxxi,
+    QG *__restrict__ dxeta, QG *__restrict__ dxzeta, double v_cut, double eosvmin,
+    double eosvmax, DevState *states) {
+    const int prob = blockProb[blockIdx.x];
+    DevState *st = states + prob;
+    const ProbDesc pd = pds[prob];
+    const int li = (blockIdx.x - pd.eBlock0) * blockDim.x + threadIdx.x;
+    if (li >= pd.E || !st->active) return;
+    const int i = pd.elemOff + li; // element index in the concatenated arrays
     const double deltaTime = st->deltatime;
+    double *const *nf = nb.b[st->cur ^ 1];
     int n[8];
     if (STRUCT) {
         // element (k,j,i) -> its eight nodes, lulesh-init.cc:236-244
-        const int nn = s + 1;
-        const int k = i / (s * s), r = i - k * s * s, j = r / s, ci = r - j * s;
-        const int b = (k * nn + j) * nn + ci;
+        const int s = pd.s, nn = s + 1;
+        const int k = li / (s * s), r = li - k * s * s, j = r / s, ci = r - j * s;
+        const int b = pd.nodeOff + (k * nn + j) * nn + ci;
         n[0] = b; n[1] = b + 1; n[2] = b + nn + 1; n[3] = b + nn;
         n[4] = b + nn * nn; n[5] = b + nn * nn + 1; n[6] = b + nn * nn + nn + 1;
         n[7] = b + nn * nn + nn;
     } else {
-        load_nodelist(nodelist, i, n);
+        load_nodelist(nodelist, li, n);
     }
     double x[8], y[8], z[8], xd[8], yd[8], zd[8];
-    load8(nf.x, n, x);
-    load8(nf.y, n, y);
-    load8(nf.z, n, z);
+    load8(nf[0], n, x);
+    load8(nf[1], n, y);
+    load8(nf[2], n, z);
     const double volume = elem_volume<double>(x, y, z);
-    const double voloi = volo[i];
+    const double voloi = f0.volo[i];
     const double rel = volume / voloi;
-    const double vold = v[i];
-    vnew[i] = rel;
-    delv[i] = rel - vold;
-    arealg[i] = characteristic_length(x, y, z, volume);
-    load8(nf.xd, n, xd);
-    load8(nf.yd, n, yd);
-    load8(nf.zd, n, zd);
+    const double vold = f0.v[i];
+    f0.vnew[i] = rel;
+    f0.delv[i] = rel - vold;
+    f0.arealg[i] = characteristic_length(x, y, z, volume);
+    load8(nf[3], n, xd);
+    load8(nf[4], n, yd);
+    load8(nf[5], n, zd);
     // Q gradients on the end-of-step geometry (lulesh.cc:1349)
     {
         QG g[6];
@@ -310,12 +328,12 @@ __global__ __launch_bounds__(kBlock) void k_elems(
             }
             q_gradients<float>(xf, yf, zf, xv, yv, zv, float(voloi * rel), g);
         }
-        delx_xi[i] = g[0];
-        delx_eta[i] = g[1];
-        delx_zeta[i] = g[2];
-        delv_xi[i] = g[3];
-        delv_eta[i] = g[4];
-        delv_zeta[i] = g[5];
+        dxxi[i] = g[0];
+        dxeta[i] = g[1];
+        dxzeta[i] = g[2];
+        dvxi[i] = g[3];
+        dveta[i] = g[4];
+        dvzeta[i] = g[5];
     }
     // Velocity divergence on the half-step geometry (lulesh.cc:1218-1239)
     const double dt2 = 0.5 * deltaTime;
@@ -327,19 +345,25 @@ __global__ __launch_bounds__(kBlock) void k_elems(
     }
     double B[3][8];
     const double detJ = shape_derivatives<true>(x, y, z, B);
-    vdov[i] = velocity_divergence(xd, yd, zd, B, detJ);
+    f0.vdov[i] = velocity_divergence(xd, yd, zd, B, detJ);
     // CheckAndUpdateVolumeForElems, lulesh.cc:1255-1270
     double vc = vold;
     if (eosvmin != 0.0 && vc < eosvmin) vc = eosvmin;
     if (eosvmax != 0.0 && vc > eosvmax) vc = eosvmax;
-    if (vc < 0.0) st->error = LULESH_VOLUME_ERROR;
-    v[i] = (fabs_(rel - 1.0) < v_cut) ? 1.0 : rel;
+    if (vc < 0.0) st->error = kVolumeError;
+    f0.v[i] = (fabs_(rel - 1.0) < v_cut) ? 1.0 : rel;
 }
 
 struct ElemFields {
     const int32_t *elemBC, *lxim, *lxip, *letam, *letap, *lzetam, *lzetap;
     const double *vdov, *elemMass, *volo, *vnew, *delv, *arealg;
     double *e, *p, *q, *ql, *qq, *ss;
+};
+
+// One segment of the material launch: the elements (concatenated indices,
+// ascending) of one problem's regions that share a repetition count.
+struct MatSeg {
+    int32_t prob, rep, offset, count, block0;
 };
 
 // Minimum over a block of two doubles, then one atomic per block.
@@ -371,23 +395,25 @@ __device__ __forceinline__ void block_min_atomic(double c, double h, DevState *s
 // ---------------------------------------------------------------------------
 // Material: CalcMonotonicQRegionForElemsAndApplyInitTask (lulesh.cc:1420-1669),
 // EvalEOSAllInOneTask repeated `rep` times (lulesh.cc:2190-2194; every
-// repetition re-reads the same inputs, so it recomputes the same values),
-// CalcSoundSpeedForElemsAndSaveTask (1873-1924), then the element's Courant
-// and hydro constraints (1938-1974) folded into the block minimum.
-// `list` holds the elements of regions sharing one repetition count.
+// repetition re-reads its inputs from the Domain, so all of them are treated
+// as new values each time), CalcSoundSpeedForElemsAndSaveTask (1873-1924),
+// then the element's Courant and hydro constraints (1938-1974) folded into
+// the block minimum.
 // ---------------------------------------------------------------------------
 template <typename QG>
 __global__ __launch_bounds__(kBlock) void k_material(
-    int32_t count, const int32_t *__restrict__ list, int32_t rep, ElemFields f,
-    const QG *__restrict__ delv_xi, const QG *__restrict__ delv_eta,
-    const QG *__restrict__ delv_zeta, const QG *__restrict__ delx_xi,
-    const QG *__restrict__ delx_eta, const QG *__restrict__ delx_zeta, Params P,
-    DevState *st) {
-    if (!st->active) return; // uniform across the grid
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const MatSeg *__restrict__ segs, const int32_t *__restrict__ blockSeg,
+    const int32_t *__restrict__ list, ElemFields f, const QG *__restrict__ delv_xi,
+    const QG *__restrict__ delv_eta, const QG *__restrict__ delv_zeta,
+    const QG *__restrict__ delx_xi, const QG *__restrict__ delx_eta,
+    const QG *__restrict__ delx_zeta, Params P, DevState *states) {
+    const MatSeg sg = segs[blockSeg[blockIdx.x]];
+    DevState *st = states + sg.prob;
+    if (!st->active) return; // uniform across the block
+    const int t = (blockIdx.x - sg.block0) * blockDim.x + threadIdx.x;
     double dtc = 1.7976931348623157e+308, dth = 1.7976931348623157e+308;
-    if (t < count) {
-        const int i = list[t];
+    if (t < sg.count) {
+        const int i = list[sg.offset + t];
         const int32_t bc = f.elemBC[i];
         const double sxi = double(delv_xi[i]), seta = double(delv_eta[i]),
                      szeta = double(delv_zeta[i]);
@@ -418,10 +444,16 @@ __global__ __launch_bounds__(kBlock) void k_material(
         if (P.eosvmax != 0.0 && vnewc > P.eosvmax) vnewc = P.eosvmax;
         const double e0 = f.e[i], dv0 = f.delv[i], p0 = f.p[i], q0 = f.q[i];
         double en = 0, pn = 0, qn = 0, ssn = 0;
-        for (int r = 0; r < rep; ++r) {
-            double ei = e0;
+        for (int r = 0; r < sg.rep; ++r) {
+            double ei = e0, di = dv0, pi = p0, qi = q0, qqi = qquad, qli = qlin, vi = vnewc;
             opaque(ei);
-            eos(ei, dv0, p0, q0, qquad, qlin, vnewc, P, en, pn, qn, ssn);
+            opaque(di);
+            opaque(pi);
+            opaque(qi);
+            opaque(qqi);
+            opaque(qli);
+            opaque(vi);
+            eos(ei, di, pi, qi, qqi, qli, vi, P, en, pn, qn, ssn);
         }
         f.ss[i] = ssn;
         f.p[i] = pn;
@@ -432,9 +464,9 @@ __global__ __launch_bounds__(kBlock) void k_material(
     block_min_atomic(dtc, dth, st);
 }
 
-// Stand-alone constraint pass over all elements (CalcConstraintForElemsTask,
-// lulesh.cc:1938-1974): the same minima as the fused path, for phase-level
-// checks on arbitrary inputs.
+// Stand-alone constraint pass over one problem's elements
+// (CalcConstraintForElemsTask, lulesh.cc:1938-1974): the same minima as the
+// fused path, for phase-level checks on arbitrary inputs.
 __global__ __launch_bounds__(kBlock) void k_constraints(int32_t E, const double *__restrict__ ss,
                                                         const double *__restrict__ vdov,
                                                         const double *__restrict__ arealg,
@@ -470,43 +502,54 @@ struct KernelTimer {
     int64_t launches = 0;
 };
 
+struct ProbHost {
+    int32_t s = 0, E = 0, N = 0, numReg = 0, cost = 0;
+    int32_t elemOff = 0, nodeOff = 0;
+    std::vector<MatSeg> segs; // offsets into matList; block0 set by configure()
+};
+
 struct lulesh_ctx {
     int precision = 0;
-    int32_t s = 0, E = 0, N = 0, numReg = 0, cost = 0;
+    bool structured = false;
+    bool store_forces = false; // also write fx.. and xdd.. (Domain fidelity)
+    int nprob = 0;
+    std::vector<ProbHost> probs;
+    int64_t totE = 0, totN = 0;
     Params P{};
     double hgcoef = 0;
     hipStream_t stream = nullptr;
     std::vector<void *> allocs;
     int64_t bytes = 0;
+    // field base pointers (concatenated over problems); the node fields
+    // x..zd live in nodeBuf (two halves on the structured path)
     double *f[LULESH_F_COUNT] = {};
-    // mixed-form FP32 gradient storage
-    float *g32[6] = {};
-    int32_t *nodelist = nullptr, *lxim = nullptr, *lxip = nullptr, *letam = nullptr,
-            *letap = nullptr, *lzetam = nullptr, *lzetap = nullptr, *elemBC = nullptr;
-    int32_t *nodeStart = nullptr, *cornerMap = nullptr, *matList = nullptr;
+    float *g32[6] = {}; // mixed form: FP32 gradient storage
+    NodeBufs nb{};
+    int32_t *lxim = nullptr, *lxip = nullptr, *letam = nullptr, *letap = nullptr,
+            *lzetam = nullptr, *lzetap = nullptr, *elemBC = nullptr, *matList = nullptr;
+    // indexed path (one problem)
+    int32_t *nodelist = nullptr, *nodeStart = nullptr, *cornerMap = nullptr;
     uint8_t *bcMask = nullptr;
     double *cs[3] = {};
     void *ch[3] = {};
-    struct Segment {
-        int32_t offset, count, rep;
-    };
-    std::vector<Segment> segs;
-    // structured Sedov cube: fused z-march nodal kernel over a double-buffered
-    // node state (x, y, z, xd, yd, zd); f[] points at the current half
-    bool structured = false;
-    bool store_forces = false; // also write fx.. and xdd.. (Domain fidelity)
-    int zplanes = 12;          // node planes per z-march workgroup
-    double *nodeBuf[2][6] = {};
-    int parity = 0;
+    // launch tables (rebuilt by configure())
+    lulesh_tuning tune{};
+    std::vector<ProbDesc> hpd;
+    std::vector<MatSeg> hsegs;
+    int zBlocks = 0, eBlocks = 0, mBlocks = 0;
+    ProbDesc *d_pd = nullptr;
+    MatSeg *d_segs = nullptr;
+    int32_t *d_zBlockProb = nullptr, *d_eBlockProb = nullptr, *d_mBlockSeg = nullptr;
+    // time states
     DevState *st = nullptr;
-    DevState *hst = nullptr; // pinned mirror
-    DevState init{};
+    DevState *hst = nullptr; // pinned mirror of all problems' states
+    std::vector<DevState> init;
     hipGraphExec_t graph = nullptr;
     int graph_cycles = 0;
-    int graph_parity = 0;
     bool timing = false;
     std::vector<KernelTimer> timers;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, run0 = nullptr, run1 = nullptr;
+    double last_run_ms = 0;
     std::string err;
 };
 
@@ -528,6 +571,7 @@ int dalloc(lulesh_ctx *c, T **p, size_t n) {
     void *q = nullptr;
     const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
     HIP_TRY(c, hipMalloc(&q, bytes));
+    HIP_TRY(c, hipMemset(q, 0, bytes));
     c->allocs.push_back(q);
     c->bytes += int64_t(bytes);
     *p = static_cast<T *>(q);
@@ -542,6 +586,7 @@ int upload(lulesh_ctx *c, T *dst, const T *src, size_t n) {
 }
 
 bool is_node_field(int id) { return id <= LULESH_F_NODALMASS; }
+bool is_state_node_field(int id) { return id >= LULESH_F_X && id <= LULESH_F_ZD; }
 bool is_grad_field(int id) { return id >= LULESH_F_DELV_XI && id <= LULESH_F_DELX_ZETA; }
 
 double *view_field(const lulesh_domain_view *v, int id) {
@@ -553,21 +598,8 @@ double *view_field(const lulesh_domain_view *v, int id) {
     return tab[id];
 }
 
-// Index of the FP32 gradient buffer for a field id (mixed form).
-int grad_slot(int id) {
-    switch (id) {
-    case LULESH_F_DELV_XI: return 0;
-    case LULESH_F_DELV_ETA: return 1;
-    case LULESH_F_DELV_ZETA: return 2;
-    case LULESH_F_DELX_XI: return 3;
-    case LULESH_F_DELX_ETA: return 4;
-    default: return 5;
-    }
-}
+int grad_slot(int id) { return id - LULESH_F_DELV_XI; }
 
-template <typename T> struct KernelSet;
-
-// Launch one kernel, optionally bracketed by events for per-kernel timing.
 template <typename F>
 int launch(lulesh_ctx *c, int timer_id, F &&fn) {
     if (c->timing) HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
@@ -584,49 +616,120 @@ int launch(lulesh_ctx *c, int timer_id, F &&fn) {
     return 0;
 }
 
-enum { T_TIME = 0, T_FORCE, T_NODES, T_ELEMS, T_MATERIAL, T_COUNT };
+enum { T_TIME = 0, T_NODAL, T_NODES, T_ELEMS, T_MATERIAL, T_COUNT };
 
-void set_node_parity(lulesh_ctx *c, int parity) {
-    c->parity = parity;
-    for (int k = 0; k < 6; ++k) c->f[LULESH_F_X + k] = c->nodeBuf[parity][k];
+// Node planes per z-march workgroup when the tuning leaves it open: halve the
+// z extent until the whole launch has about four workgroups per CU.
+int auto_zplanes(const lulesh_ctx *c) {
+    int nmax = 1;
+    for (const auto &p : c->probs) nmax = std::max(nmax, p.s + 1);
+    auto blocks = [&](int z) {
+        int64_t b = 0;
+        for (const auto &p : c->probs) {
+            const int n = p.s + 1, t = (n + kTile - 1) / kTile;
+            b += int64_t(t) * t * ((n + z - 1) / z);
+        }
+        return b;
+    };
+    int zp = nmax;
+    while (zp > 4 && blocks(zp) < 1024) zp = (zp + 1) / 2;
+    return std::max(zp, 1);
+}
+
+// Rebuild the block -> problem / segment tables for the current tuning.
+int configure(lulesh_ctx *c) {
+    lulesh_tuning &t = c->tune;
+    if (t.block_elem != 64 && t.block_elem != 128 && t.block_elem != 256) t.block_elem = 256;
+    if (t.block_material != 64 && t.block_material != 128 && t.block_material != 256)
+        t.block_material = 256;
+    if (t.zplanes <= 0) t.zplanes = auto_zplanes(c);
+    if (t.cycles_per_graph <= 0) t.cycles_per_graph = c->totE >= 1000000 ? 8 : 32;
+    std::vector<int32_t> zb, eb, mb;
+    c->hpd.clear();
+    c->hsegs.clear();
+    for (int pi = 0; pi < c->nprob; ++pi) {
+        const ProbHost &ph = c->probs[size_t(pi)];
+        ProbDesc d{};
+        d.s = ph.s;
+        d.E = ph.E;
+        d.N = ph.N;
+        d.elemOff = ph.elemOff;
+        d.nodeOff = ph.nodeOff;
+        const int n = ph.s + 1;
+        d.tiles = (n + kTile - 1) / kTile;
+        d.zplanes = std::min(t.zplanes, n);
+        d.zBlock0 = int32_t(zb.size());
+        const int64_t nz = int64_t(d.tiles) * d.tiles * ((n + d.zplanes - 1) / d.zplanes);
+        zb.insert(zb.end(), size_t(nz), pi);
+        d.eBlock0 = int32_t(eb.size());
+        eb.insert(eb.end(), size_t(grid_for(ph.E, t.block_elem)), pi);
+        c->hpd.push_back(d);
+    }
+    // material segments, the costliest (largest rep) first so they start early
+    std::vector<MatSeg> all;
+    for (const auto &ph : c->probs) all.insert(all.end(), ph.segs.begin(), ph.segs.end());
+    std::stable_sort(all.begin(), all.end(),
+                     [](const MatSeg &a, const MatSeg &b) { return a.rep > b.rep; });
+    for (auto &sg : all) {
+        if (sg.count == 0) continue;
+        sg.block0 = int32_t(mb.size());
+        mb.insert(mb.end(), size_t(grid_for(sg.count, t.block_material)),
+                  int32_t(c->hsegs.size()));
+        c->hsegs.push_back(sg);
+    }
+    c->zBlocks = int(zb.size());
+    c->eBlocks = int(eb.size());
+    c->mBlocks = int(mb.size());
+    auto put = [&](auto **dptr, const auto &vec) -> int {
+        using T = std::remove_reference_t<decltype(vec[0])>;
+        if (*dptr) (void)hipFree(*dptr);
+        *dptr = nullptr;
+        HIP_TRY(c, hipMalloc(reinterpret_cast<void **>(dptr),
+                             std::max<size_t>(vec.size(), 1) * sizeof(T)));
+        if (!vec.empty())
+            HIP_TRY(c, hipMemcpy(*dptr, vec.data(), vec.size() * sizeof(T),
+                                 hipMemcpyHostToDevice));
+        return 0;
+    };
+    int rc;
+    if ((rc = put(&c->d_pd, c->hpd)) || (rc = put(&c->d_segs, c->hsegs)) ||
+        (rc = put(&c->d_zBlockProb, zb)) || (rc = put(&c->d_eBlockProb, eb)) ||
+        (rc = put(&c->d_mBlockSeg, mb)))
+        return rc;
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    return 0;
 }
 
 template <typename R>
 int enqueue_nodal(lulesh_ctx *c) {
     if (c->structured) {
-        const int cur = c->parity, nxt = cur ^ 1;
-        double *const *a = c->nodeBuf[cur];
-        double *const *b = c->nodeBuf[nxt];
-        const NodeIn in{a[0], a[1], a[2], a[3], a[4], a[5]};
-        const NodeOut out{b[0], b[1], b[2], b[3], b[4], b[5]};
         const NodeForceOut fo{c->f[LULESH_F_FX], c->f[LULESH_F_FY], c->f[LULESH_F_FZ],
                               c->f[LULESH_F_XDD], c->f[LULESH_F_YDD], c->f[LULESH_F_ZDD]};
-        const int n = c->s + 1;
-        const int tiles = (n + kTile - 1) / kTile;
-        const dim3 grid(unsigned(tiles * tiles), unsigned((n + c->zplanes - 1) / c->zplanes));
-        int rc = launch(c, T_FORCE, [&] {
+        return launch(c, T_NODAL, [&] {
             auto kern = c->store_forces ? k_nodal_zmarch<R, true> : k_nodal_zmarch<R, false>;
-            kern<<<grid, kZmarchThreads, 0, c->stream>>>(
-                c->s, c->zplanes, in, out, fo, c->f[LULESH_F_P], c->f[LULESH_F_Q],
+            kern<<<c->zBlocks, kZmarchThreads, 0, c->stream>>>(
+                c->d_pd, c->d_zBlockProb, c->nb, fo, c->f[LULESH_F_P], c->f[LULESH_F_Q],
                 c->f[LULESH_F_VOLO], c->f[LULESH_F_V], c->f[LULESH_F_SS], c->f[LULESH_F_ELEMMASS],
                 c->f[LULESH_F_NODALMASS], c->hgcoef, c->P.u_cut, c->st);
         });
-        set_node_parity(c, nxt);
-        return rc;
     }
-    const NodeFields nf{c->f[LULESH_F_X], c->f[LULESH_F_Y], c->f[LULESH_F_Z],
-                        c->f[LULESH_F_XD], c->f[LULESH_F_YD], c->f[LULESH_F_ZD]};
-    int rc = launch(c, T_FORCE, [&] {
-        k_force<R><<<grid_for(c->E, kBlock), kBlock, 0, c->stream>>>(
-            c->E, c->nodelist, nf, c->f[LULESH_F_P], c->f[LULESH_F_Q], c->f[LULESH_F_VOLO],
+    double *const *b = c->nb.b[0];
+    const NodeFields nf{b[0], b[1], b[2], b[3], b[4], b[5]};
+    const int32_t E = int32_t(c->totE), N = int32_t(c->totN);
+    int rc = launch(c, T_NODAL, [&] {
+        k_force<R><<<grid_for(E, kBlock), kBlock, 0, c->stream>>>(
+            E, c->nodelist, nf, c->f[LULESH_F_P], c->f[LULESH_F_Q], c->f[LULESH_F_VOLO],
             c->f[LULESH_F_V], c->f[LULESH_F_SS], c->f[LULESH_F_ELEMMASS], c->hgcoef, c->cs[0],
             c->cs[1], c->cs[2], static_cast<R *>(c->ch[0]), static_cast<R *>(c->ch[1]),
             static_cast<R *>(c->ch[2]), c->st);
     });
     if (rc) return rc;
     return launch(c, T_NODES, [&] {
-        k_nodes<R><<<grid_for(c->N, kBlock), kBlock, 0, c->stream>>>(
-            c->N, c->nodeStart, c->cornerMap, c->cs[0], c->cs[1], c->cs[2],
+        k_nodes<R><<<grid_for(N, kBlock), kBlock, 0, c->stream>>>(
+            N, c->nodeStart, c->cornerMap, c->cs[0], c->cs[1], c->cs[2],
             static_cast<const R *>(c->ch[0]), static_cast<const R *>(c->ch[1]),
             static_cast<const R *>(c->ch[2]), c->f[LULESH_F_NODALMASS], c->bcMask, nf,
             c->f[LULESH_F_FX], c->f[LULESH_F_FY], c->f[LULESH_F_FZ], c->f[LULESH_F_XDD],
@@ -635,43 +738,40 @@ int enqueue_nodal(lulesh_ctx *c) {
 }
 
 template <typename R>
+R *grad_ptr(lulesh_ctx *c, int k) {
+    if constexpr (sizeof(R) == 8)
+        return c->f[LULESH_F_DELV_XI + k];
+    else
+        return c->g32[k];
+}
+
+template <typename R>
 int enqueue_elements(lulesh_ctx *c) {
-    const NodeFields nf{c->f[LULESH_F_X], c->f[LULESH_F_Y], c->f[LULESH_F_Z],
-                        c->f[LULESH_F_XD], c->f[LULESH_F_YD], c->f[LULESH_F_ZD]};
-    R *g[6];
-    for (int k = 0; k < 6; ++k)
-        g[k] = sizeof(R) == 8 ? reinterpret_cast<R *>(c->f[LULESH_F_DELV_XI + k])
-                              : reinterpret_cast<R *>(c->g32[k]);
+    const ElemKin ek{c->f[LULESH_F_VOLO], c->f[LULESH_F_V], c->f[LULESH_F_VNEW],
+                     c->f[LULESH_F_DELV], c->f[LULESH_F_AREALG], c->f[LULESH_F_VDOV]};
     return launch(c, T_ELEMS, [&] {
         auto kern = c->structured ? k_elems<R, true> : k_elems<R, false>;
-        kern<<<grid_for(c->E, kBlock), kBlock, 0, c->stream>>>(
-            c->E, c->s, c->nodelist, nf, c->f[LULESH_F_VOLO], c->f[LULESH_F_V], c->f[LULESH_F_VNEW],
-            c->f[LULESH_F_DELV], c->f[LULESH_F_AREALG], c->f[LULESH_F_VDOV], g[0], g[1], g[2],
-            g[3], g[4], g[5], c->P.v_cut, c->P.eosvmin, c->P.eosvmax, c->st);
+        kern<<<c->eBlocks, c->tune.block_elem, 0, c->stream>>>(
+            c->d_pd, c->d_eBlockProb, c->nodelist, c->nb, ek, grad_ptr<R>(c, 0),
+            grad_ptr<R>(c, 1), grad_ptr<R>(c, 2), grad_ptr<R>(c, 3), grad_ptr<R>(c, 4),
+            grad_ptr<R>(c, 5), c->P.v_cut, c->P.eosvmin, c->P.eosvmax, c->st);
     });
 }
 
 template <typename R>
 int enqueue_material(lulesh_ctx *c) {
+    if (c->mBlocks == 0) return 0;
     const ElemFields ef{c->elemBC, c->lxim, c->lxip, c->letam, c->letap, c->lzetam, c->lzetap,
                         c->f[LULESH_F_VDOV], c->f[LULESH_F_ELEMMASS], c->f[LULESH_F_VOLO],
                         c->f[LULESH_F_VNEW], c->f[LULESH_F_DELV], c->f[LULESH_F_AREALG],
                         c->f[LULESH_F_E], c->f[LULESH_F_P], c->f[LULESH_F_Q], c->f[LULESH_F_QL],
                         c->f[LULESH_F_QQ], c->f[LULESH_F_SS]};
-    const R *g[6];
-    for (int k = 0; k < 6; ++k)
-        g[k] = sizeof(R) == 8 ? reinterpret_cast<const R *>(c->f[LULESH_F_DELV_XI + k])
-                              : reinterpret_cast<const R *>(c->g32[k]);
-    for (const auto &sg : c->segs) {
-        if (sg.count == 0) continue;
-        int rc = launch(c, T_MATERIAL, [&] {
-            k_material<R><<<grid_for(sg.count, kBlock), kBlock, 0, c->stream>>>(
-                sg.count, c->matList + sg.offset, sg.rep, ef, g[0], g[1], g[2], g[3], g[4], g[5],
-                c->P, c->st);
-        });
-        if (rc) return rc;
-    }
-    return 0;
+    return launch(c, T_MATERIAL, [&] {
+        k_material<R><<<c->mBlocks, c->tune.block_material, 0, c->stream>>>(
+            c->d_segs, c->d_mBlockSeg, c->matList, ef, grad_ptr<R>(c, 0), grad_ptr<R>(c, 1),
+            grad_ptr<R>(c, 2), grad_ptr<R>(c, 3), grad_ptr<R>(c, 4), grad_ptr<R>(c, 5), c->P,
+            c->st);
+    });
 }
 
 int enqueue_cycle_body(lulesh_ctx *c) {
@@ -687,16 +787,16 @@ int enqueue_cycle_body(lulesh_ctx *c) {
 }
 
 int enqueue_cycle(lulesh_ctx *c) {
-    int rc = launch(c, T_TIME, [&] { k_time_increment<<<1, 1, 0, c->stream>>>(c->st); });
+    int rc = launch(c, T_TIME, [&] {
+        k_time_increment<<<grid_for(c->nprob, 64), 64, 0, c->stream>>>(c->st, c->nprob);
+    });
     if (rc) return rc;
     return enqueue_cycle_body(c);
 }
 
-int build_graph(lulesh_ctx *c, int cycles) {
-    // an even number of cycles returns the node double buffer to the parity
-    // the graph was captured with
-    cycles += cycles & 1;
-    if (c->graph && c->graph_cycles == cycles && c->graph_parity == c->parity) return 0;
+int build_graph(lulesh_ctx *c) {
+    const int cycles = c->tune.cycles_per_graph;
+    if (c->graph && c->graph_cycles == cycles) return 0;
     if (c->graph) {
         (void)hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
@@ -715,19 +815,66 @@ int build_graph(lulesh_ctx *c, int cycles) {
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return hip_fail(c, e, "hipGraphInstantiate");
     c->graph_cycles = cycles;
-    c->graph_parity = c->parity;
     return 0;
 }
 
-int read_state(lulesh_ctx *c) {
-    HIP_TRY(c, hipMemcpyAsync(c->hst, c->st, sizeof(DevState), hipMemcpyDeviceToHost, c->stream));
+int read_states(lulesh_ctx *c) {
+    HIP_TRY(c, hipMemcpyAsync(c->hst, c->st, sizeof(DevState) * size_t(c->nprob),
+                              hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 
-} // namespace
+int write_states(lulesh_ctx *c) {
+    HIP_TRY(c, hipMemcpyAsync(c->st, c->hst, sizeof(DevState) * size_t(c->nprob),
+                              hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
 
-namespace {
+bool still_running(const DevState &h) {
+    return !h.error && h.time < h.stoptime && h.cycle < h.its;
+}
+
+// Replay cycle graphs (or time every launch) until no problem runs.
+int run_loop(lulesh_ctx *c) {
+    int rc = 0;
+    if (!c->timing && (rc = build_graph(c))) return rc;
+    const int per = c->tune.cycles_per_graph;
+    HIP_TRY(c, hipEventRecord(c->run0, c->stream));
+    for (;;) {
+        if (c->timing) {
+            for (int k = 0; k < per && rc == 0; ++k) rc = enqueue_cycle(c);
+            if (rc) return rc;
+        } else {
+            HIP_TRY(c, hipGraphLaunch(c->graph, c->stream));
+        }
+        if ((rc = read_states(c))) return rc;
+        bool any = false;
+        for (int pi = 0; pi < c->nprob; ++pi) any = any || still_running(c->hst[pi]);
+        if (!any) break;
+    }
+    HIP_TRY(c, hipEventRecord(c->run1, c->stream));
+    HIP_TRY(c, hipEventSynchronize(c->run1));
+    float ms = 0;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->run0, c->run1));
+    c->last_run_ms = ms;
+    for (int pi = 0; pi < c->nprob; ++pi)
+        if (c->hst[pi].error) {
+            c->err = "VolumeError in problem " + std::to_string(pi);
+            return c->hst[pi].error;
+        }
+    return 0;
+}
+
+void to_time_state(const DevState &h, lulesh_time_state *ts) {
+    ts->time = h.time;
+    ts->deltatime = h.deltatime;
+    ts->dtcourant = bits_to_double(h.dtc_bits);
+    ts->dthydro = bits_to_double(h.dth_bits);
+    ts->cycle = h.cycle;
+    ts->status = h.error;
+}
 
 // True when the Domain is the structured Sedov cube of the reference set-up
 // (lulesh-init.cc:202-310, 448-468): canonical connectivity, corner lists in
@@ -764,9 +911,9 @@ bool is_structured_cube(const lulesh_domain_view *v) {
         want.reserve(size_t(P));
         for (int64_t a = 0; a < n; ++a)
             for (int64_t b = 0; b < n; ++b)
-                want.push_back(int32_t(axis == 0 ? (a * n + b) * n        // I == 0
-                                      : axis == 1 ? (a * n) * n + b       // J == 0
-                                                  : a * n + b));          // K == 0
+                want.push_back(int32_t(axis == 0 ? (a * n + b) * n   // I == 0
+                                      : axis == 1 ? (a * n) * n + b  // J == 0
+                                                  : a * n + b));     // K == 0
         std::sort(got.begin(), got.end());
         std::sort(want.begin(), want.end());
         if (got != want) return false;
@@ -774,160 +921,23 @@ bool is_structured_cube(const lulesh_domain_view *v) {
     return true;
 }
 
-} // namespace
+bool view_ok(const lulesh_domain_view *v) {
+    return v && v->sizeX >= 1 && v->numElem == v->sizeX * v->sizeX * v->sizeX &&
+           v->numNode == (v->sizeX + 1) * (v->sizeX + 1) * (v->sizeX + 1) && v->numReg >= 1 &&
+           v->nodelist && v->nodeElemStart && v->nodeElemCornerList && v->regElemSize &&
+           v->regElemlist && v->elemBC && v->lxim && v->lxip && v->letam && v->letap &&
+           v->lzetam && v->lzetap && v->x && v->y && v->z && v->xd && v->yd && v->zd && v->e &&
+           v->p && v->q && v->v && v->ss && v->nodalMass && v->volo && v->elemMass &&
+           int64_t(v->numElem) * 8 < (int64_t(1) << 31);
+}
 
-extern "C" {
-
-int lulesh_hip_create(const lulesh_domain_view *v, int precision_opts, lulesh_ctx **out) {
-    const int precision = precision_opts & 0xff;
-    if (precision_opts & ~(0xff | LULESH_OPT_NODE_FORCES | LULESH_OPT_GENERIC))
-        return LULESH_INVALID_ARGUMENT;
-    if (!v || !out || v->sizeX < 1 || v->numElem != v->sizeX * v->sizeX * v->sizeX ||
-        v->numNode != (v->sizeX + 1) * (v->sizeX + 1) * (v->sizeX + 1) || v->numReg < 1 ||
-        (precision != LULESH_PRECISION_FP64 && precision != LULESH_PRECISION_MIXED) ||
-        !v->nodelist || !v->nodeElemStart || !v->nodeElemCornerList || !v->regElemSize ||
-        !v->regElemlist || !v->elemBC || !v->x || !v->e || !v->nodalMass || !v->volo ||
-        !v->elemMass)
-        return LULESH_INVALID_ARGUMENT;
-    if (int64_t(v->numElem) * 8 >= int64_t(1) << 31) return LULESH_INVALID_ARGUMENT;
-    *out = nullptr;
-    auto *c = new lulesh_ctx();
-    c->precision = precision;
-    c->store_forces = (precision_opts & LULESH_OPT_NODE_FORCES) != 0;
-    c->structured = !(precision_opts & LULESH_OPT_GENERIC) && is_structured_cube(v);
-    c->s = v->sizeX;
-    c->E = v->numElem;
-    c->N = v->numNode;
-    c->numReg = v->numReg;
-    c->cost = v->cost;
-    c->hgcoef = v->hgcoef;
-    c->P = Params{v->e_cut, v->p_cut, v->q_cut, v->v_cut, v->u_cut, v->hgcoef,
+Params params_of(const lulesh_domain_view *v) {
+    return Params{v->e_cut, v->p_cut, v->q_cut, v->v_cut, v->u_cut, v->hgcoef,
                   v->monoq_max_slope, v->monoq_limiter_mult, v->qlc_monoq, v->qqc_monoq, v->qqc,
                   v->eosvmax, v->eosvmin, v->pmin, v->emin, v->dvovmax, v->refdens};
-    c->timers = {{"time_increment"}, {"force"}, {"nodes"}, {"elements"}, {"material"}};
-    const size_t E = size_t(c->E), N = size_t(c->N);
-    auto fail = [&](int rc) {
-        lulesh_hip_destroy(c);
-        return rc;
-    };
-    int rc;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
-        return fail(LULESH_HIP_ERROR);
-    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
-        return fail(LULESH_HIP_ERROR);
-    for (int id = 0; id < LULESH_F_COUNT; ++id) {
-        const size_t n = is_node_field(id) ? N : E;
-        if (precision == LULESH_PRECISION_MIXED && is_grad_field(id)) {
-            if ((rc = dalloc(c, &c->g32[grad_slot(id)], n))) return fail(rc);
-            continue;
-        }
-        if ((rc = dalloc(c, &c->f[id], n))) return fail(rc);
-        if (hipMemset(c->f[id], 0, n * sizeof(double)) != hipSuccess) return fail(LULESH_HIP_ERROR);
-        if ((rc = upload(c, c->f[id], (const double *)view_field(v, id), n))) return fail(rc);
-    }
-    for (int k = 0; k < 6; ++k) c->nodeBuf[0][k] = c->f[LULESH_F_X + k];
-    if (c->structured) {
-        for (int k = 0; k < 6; ++k)
-            if ((rc = dalloc(c, &c->nodeBuf[1][k], N))) return fail(rc);
-    } else if ((rc = dalloc(c, &c->nodelist, 8 * E)) ||
-               (rc = upload(c, c->nodelist, v->nodelist, 8 * E))) {
-        return fail(rc);
-    }
-    int32_t **conn[7] = {&c->lxim, &c->lxip, &c->letam, &c->letap, &c->lzetam, &c->lzetap,
-                         &c->elemBC};
-    const int32_t *hconn[7] = {v->lxim, v->lxip, v->letam, v->letap, v->lzetam, v->lzetap,
-                               v->elemBC};
-    for (int k = 0; k < 7; ++k) {
-        if (!hconn[k]) return fail(LULESH_INVALID_ARGUMENT);
-        if ((rc = dalloc(c, conn[k], E)) || (rc = upload(c, *conn[k], hconn[k], E)))
-            return fail(rc);
-    }
-    // Corner CSR with corner ids remapped to the corner-major layout of the
-    // force arrays: corner 8*e+k lives at k*E+e (indexed path only).
-    if (!c->structured) {
-        std::vector<int32_t> cmap(8 * E);
-        for (size_t i = 0; i < 8 * E; ++i) {
-            const int32_t cl = v->nodeElemCornerList[i];
-            cmap[i] = (cl & 7) * c->E + (cl >> 3);
-        }
-        if ((rc = dalloc(c, &c->nodeStart, N + 1)) ||
-            (rc = upload(c, c->nodeStart, v->nodeElemStart, N + 1)) ||
-            (rc = dalloc(c, &c->cornerMap, 8 * E)) ||
-            (rc = upload(c, c->cornerMap, cmap.data(), 8 * E)))
-            return fail(rc);
-    }
-    // Symmetry-plane node sets as a per-node mask (bit0 x, bit1 y, bit2 z).
-    if (!c->structured) {
-        std::vector<uint8_t> mask(N, 0);
-        const size_t P = size_t(c->s + 1) * (c->s + 1);
-        const int32_t *sets[3] = {v->symmX, v->symmY, v->symmZ};
-        for (int a = 0; a < 3; ++a)
-            if (sets[a])
-                for (size_t k = 0; k < P; ++k) mask[size_t(sets[a][k])] |= uint8_t(1u << a);
-        if ((rc = dalloc(c, &c->bcMask, N)) || (rc = upload(c, c->bcMask, mask.data(), N)))
-            return fail(rc);
-    }
-    // Region lists grouped by EOS repetition count (lulesh.cc:2159-2169);
-    // within a group, elements ascend.
-    {
-        std::vector<int32_t> reps(size_t(c->numReg));
-        for (int r = 0; r < c->numReg; ++r) reps[size_t(r)] = region_rep(r, c->numReg, c->cost);
-        std::vector<int32_t> uniq(reps);
-        std::sort(uniq.begin(), uniq.end());
-        uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
-        std::vector<int32_t> all;
-        all.reserve(E);
-        for (int32_t rv : uniq) {
-            const int32_t off = int32_t(all.size());
-            for (int r = 0; r < c->numReg; ++r)
-                if (reps[size_t(r)] == rv)
-                    all.insert(all.end(), v->regElemlist[r], v->regElemlist[r] + v->regElemSize[r]);
-            std::sort(all.begin() + off, all.end());
-            c->segs.push_back({off, int32_t(all.size()) - off, rv});
-        }
-        if (all.size() != E) return fail(LULESH_INVALID_ARGUMENT);
-        if ((rc = dalloc(c, &c->matList, E)) || (rc = upload(c, c->matList, all.data(), E)))
-            return fail(rc);
-    }
-    // Corner force arrays (allocated once; the reference mallocs them every
-    // cycle, lulesh.cc:2017-2022).
-    for (int k = 0; k < 3 && !c->structured; ++k) {
-        if ((rc = dalloc(c, &c->cs[k], 8 * E))) return fail(rc);
-        if (precision == LULESH_PRECISION_MIXED) {
-            float *p = nullptr;
-            if ((rc = dalloc(c, &p, 8 * E))) return fail(rc);
-            c->ch[k] = p;
-        } else {
-            double *p = nullptr;
-            if ((rc = dalloc(c, &p, 8 * E))) return fail(rc);
-            c->ch[k] = p;
-        }
-    }
-    // Mixed form: narrow any provided gradients into FP32 storage.
-    if (precision == LULESH_PRECISION_MIXED) {
-        for (int id = LULESH_F_DELV_XI; id <= LULESH_F_DELX_ZETA; ++id) {
-            const double *src = view_field(v, id);
-            if (!src) continue;
-            double *tmp = nullptr;
-            if (hipMalloc(&tmp, E * sizeof(double)) != hipSuccess) return fail(LULESH_HIP_ERROR);
-            (void)hipMemcpy(tmp, src, E * sizeof(double), hipMemcpyHostToDevice);
-            k_narrow<float><<<grid_for(c->E, 256), 256, 0, c->stream>>>(c->E, tmp,
-                                                                        c->g32[grad_slot(id)]);
-            (void)hipStreamSynchronize(c->stream);
-            (void)hipFree(tmp);
-        }
-    }
-    if (c->structured) {
-        // z extent of a z-march workgroup: enough workgroups to fill the chip
-        // (about 4 per CU) while keeping the one-plane halo small.
-        const int n = c->s + 1, tiles = (n + kTile - 1) / kTile;
-        int zp = n;
-        while (zp > 4 && int64_t(tiles) * tiles * ((n + zp - 1) / zp) < 1024) zp = (zp + 1) / 2;
-        c->zplanes = std::max(zp, 1);
-    }
-    if ((rc = dalloc(c, &c->st, 1))) return fail(rc);
-    if (hipHostMalloc(reinterpret_cast<void **>(&c->hst), sizeof(DevState)) != hipSuccess)
-        return fail(LULESH_HIP_ERROR);
+}
+
+DevState state_of(const lulesh_domain_view *v) {
     DevState s{};
     s.time = v->time;
     s.deltatime = v->deltatime;
@@ -940,22 +950,283 @@ int lulesh_hip_create(const lulesh_domain_view *v, int precision_opts, lulesh_ct
     s.dth_bits = host_dbits(v->dthydro);
     s.cycle = v->cycle;
     s.its = v->cycle;
-    s.active = 0;
-    s.error = 0;
-    c->init = s;
-    if (hipMemcpy(c->st, &s, sizeof s, hipMemcpyHostToDevice) != hipSuccess)
+    return s;
+}
+
+int create_impl(const lulesh_domain_view *const *views, int nprob, int precision_opts,
+                lulesh_ctx **out) {
+    if (!out) return LULESH_INVALID_ARGUMENT;
+    *out = nullptr;
+    const int precision = precision_opts & 0xff;
+    if ((precision_opts & ~(0xff | LULESH_OPT_NODE_FORCES | LULESH_OPT_GENERIC)) ||
+        (precision != LULESH_PRECISION_FP64 && precision != LULESH_PRECISION_MIXED) ||
+        nprob < 1 || !views)
+        return LULESH_INVALID_ARGUMENT;
+    int64_t totE = 0, totN = 0;
+    for (int pi = 0; pi < nprob; ++pi) {
+        if (!view_ok(views[pi])) return LULESH_INVALID_ARGUMENT;
+        totE += views[pi]->numElem;
+        totN += views[pi]->numNode;
+        if (std::memcmp(&views[pi]->e_cut, &views[0]->e_cut, sizeof(double) * 19) != 0)
+            return LULESH_INVALID_ARGUMENT; // run constants are shared by the kernels
+    }
+    if (totE * 8 >= (int64_t(1) << 31) || totN >= (int64_t(1) << 31))
+        return LULESH_INVALID_ARGUMENT;
+    bool structured = !(precision_opts & LULESH_OPT_GENERIC);
+    for (int pi = 0; pi < nprob && structured; ++pi) structured = is_structured_cube(views[pi]);
+    if (!structured && nprob > 1) return LULESH_INVALID_ARGUMENT; // batches need cubes
+
+    auto *c = new lulesh_ctx();
+    auto fail = [&](int rc) {
+        lulesh_hip_destroy(c);
+        return rc;
+    };
+    c->precision = precision;
+    c->store_forces = (precision_opts & LULESH_OPT_NODE_FORCES) != 0;
+    c->structured = structured;
+    c->nprob = nprob;
+    c->totE = totE;
+    c->totN = totN;
+    c->P = params_of(views[0]);
+    c->hgcoef = views[0]->hgcoef;
+    c->timers = {{"time_increment"}, {"nodal"}, {"nodes"}, {"elements"}, {"material"}};
+    int rc;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->run0) != hipSuccess || hipEventCreate(&c->run1) != hipSuccess)
         return fail(LULESH_HIP_ERROR);
+    {
+        int32_t eo = 0, no = 0;
+        for (int pi = 0; pi < nprob; ++pi) {
+            ProbHost ph;
+            ph.s = views[pi]->sizeX;
+            ph.E = views[pi]->numElem;
+            ph.N = views[pi]->numNode;
+            ph.numReg = views[pi]->numReg;
+            ph.cost = views[pi]->cost;
+            ph.elemOff = eo;
+            ph.nodeOff = no;
+            eo += ph.E;
+            no += ph.N;
+            c->probs.push_back(ph);
+        }
+    }
+    const size_t E = size_t(totE), N = size_t(totN);
+    // Fields, concatenated over problems.
+    for (int id = 0; id < LULESH_F_COUNT; ++id) {
+        const size_t n = is_node_field(id) ? N : E;
+        if (precision == LULESH_PRECISION_MIXED && is_grad_field(id)) {
+            if ((rc = dalloc(c, &c->g32[grad_slot(id)], n))) return fail(rc);
+            continue;
+        }
+        if ((rc = dalloc(c, &c->f[id], n))) return fail(rc);
+        for (int pi = 0; pi < nprob; ++pi) {
+            const ProbHost &ph = c->probs[size_t(pi)];
+            const size_t off = is_node_field(id) ? size_t(ph.nodeOff) : size_t(ph.elemOff);
+            const size_t cnt = is_node_field(id) ? size_t(ph.N) : size_t(ph.E);
+            if ((rc = upload(c, c->f[id] + off, (const double *)view_field(views[pi], id), cnt)))
+                return fail(rc);
+        }
+    }
+    for (int k = 0; k < 6; ++k) {
+        c->nb.b[0][k] = c->f[LULESH_F_X + k];
+        c->nb.b[1][k] = c->f[LULESH_F_X + k];
+        if (structured && (rc = dalloc(c, &c->nb.b[1][k], N))) return fail(rc);
+    }
+    // Face neighbours and BC flags, shifted to concatenated element indices.
+    {
+        int32_t **dst[7] = {&c->lxim, &c->lxip, &c->letam, &c->letap, &c->lzetam, &c->lzetap,
+                            &c->elemBC};
+        std::vector<int32_t> buf(E);
+        for (int k = 0; k < 7; ++k) {
+            for (int pi = 0; pi < nprob; ++pi) {
+                const lulesh_domain_view *v = views[pi];
+                const int32_t *src[7] = {v->lxim, v->lxip, v->letam, v->letap,
+                                         v->lzetam, v->lzetap, v->elemBC};
+                const int32_t off = c->probs[size_t(pi)].elemOff;
+                for (int32_t i = 0; i < v->numElem; ++i)
+                    buf[size_t(off + i)] = k < 6 ? src[k][i] + off : src[k][i];
+            }
+            if ((rc = dalloc(c, dst[k], E)) || (rc = upload(c, *dst[k], buf.data(), E)))
+                return fail(rc);
+        }
+    }
+    // Region lists grouped by EOS repetition count (lulesh.cc:2159-2169),
+    // ascending within a group, as concatenated element indices.
+    {
+        std::vector<int32_t> all;
+        all.reserve(E);
+        for (int pi = 0; pi < nprob; ++pi) {
+            const lulesh_domain_view *v = views[pi];
+            ProbHost &ph = c->probs[size_t(pi)];
+            std::vector<int32_t> reps(size_t(v->numReg));
+            for (int r = 0; r < v->numReg; ++r) reps[size_t(r)] = region_rep(r, v->numReg, v->cost);
+            std::vector<int32_t> uniq(reps);
+            std::sort(uniq.begin(), uniq.end());
+            uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+            int64_t seen = 0;
+            for (int32_t rv : uniq) {
+                const int32_t off = int32_t(all.size());
+                for (int r = 0; r < v->numReg; ++r)
+                    if (reps[size_t(r)] == rv)
+                        for (int32_t k = 0; k < v->regElemSize[r]; ++k)
+                            all.push_back(v->regElemlist[r][k] + ph.elemOff);
+                std::sort(all.begin() + off, all.end());
+                ph.segs.push_back({pi, rv, off, int32_t(all.size()) - off, 0});
+                seen += int32_t(all.size()) - off;
+            }
+            if (seen != ph.E) return fail(LULESH_INVALID_ARGUMENT);
+        }
+        if ((rc = dalloc(c, &c->matList, E)) || (rc = upload(c, c->matList, all.data(), E)))
+            return fail(rc);
+    }
+    if (!structured) {
+        const lulesh_domain_view *v = views[0];
+        if ((rc = dalloc(c, &c->nodelist, 8 * E)) ||
+            (rc = upload(c, c->nodelist, v->nodelist, 8 * E)))
+            return fail(rc);
+        // Corner CSR with corner ids remapped to the corner-major layout of
+        // the force arrays: corner 8*e+k lives at k*E+e.
+        std::vector<int32_t> cmap(8 * E);
+        for (size_t i = 0; i < 8 * E; ++i) {
+            const int32_t cl = v->nodeElemCornerList[i];
+            cmap[i] = (cl & 7) * int32_t(E) + (cl >> 3);
+        }
+        if ((rc = dalloc(c, &c->nodeStart, N + 1)) ||
+            (rc = upload(c, c->nodeStart, v->nodeElemStart, N + 1)) ||
+            (rc = dalloc(c, &c->cornerMap, 8 * E)) ||
+            (rc = upload(c, c->cornerMap, cmap.data(), 8 * E)))
+            return fail(rc);
+        // Symmetry-plane node sets as a per-node mask (bit0 x, bit1 y, bit2 z).
+        std::vector<uint8_t> mask(N, 0);
+        const size_t P = size_t(v->sizeX + 1) * (v->sizeX + 1);
+        const int32_t *sets[3] = {v->symmX, v->symmY, v->symmZ};
+        for (int a = 0; a < 3; ++a)
+            if (sets[a])
+                for (size_t k = 0; k < P; ++k) mask[size_t(sets[a][k])] |= uint8_t(1u << a);
+        if ((rc = dalloc(c, &c->bcMask, N)) || (rc = upload(c, c->bcMask, mask.data(), N)))
+            return fail(rc);
+        for (int k = 0; k < 3; ++k) {
+            if ((rc = dalloc(c, &c->cs[k], 8 * E))) return fail(rc);
+            if (precision == LULESH_PRECISION_MIXED) {
+                float *p = nullptr;
+                if ((rc = dalloc(c, &p, 8 * E))) return fail(rc);
+                c->ch[k] = p;
+            } else {
+                double *p = nullptr;
+                if ((rc = dalloc(c, &p, 8 * E))) return fail(rc);
+                c->ch[k] = p;
+            }
+        }
+    }
+    // Mixed form: narrow any provided gradients into FP32 storage.
+    if (precision == LULESH_PRECISION_MIXED) {
+        double *tmp = nullptr;
+        if (hipMalloc(&tmp, E * sizeof(double)) != hipSuccess) return fail(LULESH_HIP_ERROR);
+        for (int id = LULESH_F_DELV_XI; id <= LULESH_F_DELX_ZETA; ++id) {
+            (void)hipMemset(tmp, 0, E * sizeof(double));
+            for (int pi = 0; pi < nprob; ++pi)
+                if (const double *src = view_field(views[pi], id))
+                    (void)hipMemcpy(tmp + c->probs[size_t(pi)].elemOff, src,
+                                    size_t(c->probs[size_t(pi)].E) * sizeof(double),
+                                    hipMemcpyHostToDevice);
+            k_narrow<float><<<grid_for(int64_t(E), 256), 256, 0, c->stream>>>(
+                int32_t(E), tmp, c->g32[grad_slot(id)]);
+        }
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(tmp);
+    }
+    if ((rc = dalloc(c, &c->st, size_t(nprob)))) return fail(rc);
+    if (hipHostMalloc(reinterpret_cast<void **>(&c->hst), sizeof(DevState) * size_t(nprob)) !=
+        hipSuccess)
+        return fail(LULESH_HIP_ERROR);
+    for (int pi = 0; pi < nprob; ++pi) c->init.push_back(state_of(views[pi]));
+    if (hipMemcpy(c->st, c->init.data(), sizeof(DevState) * size_t(nprob),
+                  hipMemcpyHostToDevice) != hipSuccess)
+        return fail(LULESH_HIP_ERROR);
+    if ((rc = configure(c))) return fail(rc);
     if (hipDeviceSynchronize() != hipSuccess) return fail(LULESH_HIP_ERROR);
     *out = c;
     return 0;
 }
 
+// Host <-> device copy of one problem's field.  Node state fields live in
+// the half of the double buffer named by the problem's DevState.
+int field_copy(lulesh_ctx *c, int prob, int id, void *host, size_t count, bool down) {
+    if (!c || !host || id < 0 || id >= LULESH_F_COUNT || prob < 0 || prob >= c->nprob)
+        return LULESH_INVALID_ARGUMENT;
+    const ProbHost &ph = c->probs[size_t(prob)];
+    const size_t n = is_node_field(id) ? size_t(ph.N) : size_t(ph.E);
+    const size_t off = is_node_field(id) ? size_t(ph.nodeOff) : size_t(ph.elemOff);
+    if (count > n) return LULESH_INVALID_ARGUMENT;
+    int rc = read_states(c);
+    if (rc) return rc;
+    if (c->precision == LULESH_PRECISION_MIXED && is_grad_field(id)) {
+        double *tmp = nullptr;
+        HIP_TRY(c, hipMalloc(&tmp, std::max<size_t>(count, 1) * sizeof(double)));
+        hipError_t e;
+        float *g = c->g32[grad_slot(id)] + off;
+        if (down) {
+            k_widen<float><<<grid_for(int64_t(count), 256), 256, 0, c->stream>>>(int32_t(count),
+                                                                                 g, tmp);
+            e = hipMemcpyAsync(host, tmp, count * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream);
+        } else {
+            e = hipMemcpyAsync(tmp, host, count * sizeof(double), hipMemcpyHostToDevice,
+                               c->stream);
+            k_narrow<float><<<grid_for(int64_t(count), 256), 256, 0, c->stream>>>(int32_t(count),
+                                                                                  tmp, g);
+        }
+        const hipError_t e2 = hipStreamSynchronize(c->stream);
+        (void)hipFree(tmp);
+        if (e != hipSuccess) return hip_fail(c, e, "field copy");
+        if (e2 != hipSuccess) return hip_fail(c, e2, "field copy");
+        return 0;
+    }
+    double *base = c->f[id];
+    if (is_state_node_field(id)) {
+        const DevState &h = c->hst[prob];
+        base = c->nb.b[h.cur ^ (h.active ? 1 : 0)][id - LULESH_F_X];
+    }
+    double *dev = base + off;
+    HIP_TRY(c, hipMemcpyAsync(down ? host : static_cast<void *>(dev),
+                              down ? static_cast<const void *>(dev) : host,
+                              count * sizeof(double),
+                              down ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// Copies of every allocation that holds state, for the autotuner's restore.
+struct Snapshot {
+    std::vector<std::pair<void *, size_t>> regions;
+    std::vector<void *> copies;
+};
+
+} // namespace
+
+extern "C" {
+
+int lulesh_hip_create(const lulesh_domain_view *v, int precision_opts, lulesh_ctx **out) {
+    const lulesh_domain_view *views[1] = {v};
+    return create_impl(views, 1, precision_opts, out);
+}
+
+int lulesh_hip_batch_create(const lulesh_domain_view *const *views, int32_t nprob,
+                            int precision_opts, lulesh_ctx **out) {
+    return create_impl(views, nprob, precision_opts, out);
+}
+
+int32_t lulesh_hip_num_problems(const lulesh_ctx *c) { return c ? c->nprob : 0; }
+
 int lulesh_hip_leapfrog(lulesh_ctx *c, double deltatime, double *dtcourant, double *dthydro) {
-    if (!c) return LULESH_INVALID_ARGUMENT;
-    int rc = launch(c, T_TIME, [&] { k_begin_cycle<<<1, 1, 0, c->stream>>>(c->st, deltatime, 1); });
+    if (!c || c->nprob != 1) return LULESH_INVALID_ARGUMENT;
+    int rc = launch(c, T_TIME, [&] {
+        k_begin_cycle<<<1, 1, 0, c->stream>>>(c->st, deltatime, 1, 1);
+    });
     if (rc) return rc;
     if ((rc = enqueue_cycle_body(c))) return rc;
-    if ((rc = read_state(c))) return rc;
+    if ((rc = read_states(c))) return rc;
     if (c->hst->error) {
         c->err = "VolumeError";
         return c->hst->error;
@@ -966,10 +1237,13 @@ int lulesh_hip_leapfrog(lulesh_ctx *c, double deltatime, double *dtcourant, doub
 }
 
 int lulesh_hip_phase(lulesh_ctx *c, int phase, double deltatime) {
-    if (!c || phase < LULESH_PHASE_NODAL || phase > LULESH_PHASE_CONSTRAINTS)
+    if (!c || c->nprob != 1 || phase < LULESH_PHASE_NODAL || phase > LULESH_PHASE_CONSTRAINTS)
         return LULESH_INVALID_ARGUMENT;
+    // the nodal phase starts a new cycle (and adopts the previous node
+    // state); the later phases belong to the same cycle
     int rc = launch(c, T_TIME, [&] {
-        k_begin_cycle<<<1, 1, 0, c->stream>>>(c->st, deltatime, phase >= LULESH_PHASE_MATERIAL);
+        k_begin_cycle<<<1, 1, 0, c->stream>>>(c->st, deltatime, phase >= LULESH_PHASE_MATERIAL,
+                                              phase == LULESH_PHASE_NODAL);
     });
     if (rc) return rc;
     const bool mixed = c->precision == LULESH_PRECISION_MIXED;
@@ -982,18 +1256,19 @@ int lulesh_hip_phase(lulesh_ctx *c, int phase, double deltatime) {
         rc = mixed ? enqueue_material<float>(c) : enqueue_material<double>(c);
         break;
     default:
-        k_constraints<<<grid_for(c->E, kBlock), kBlock, 0, c->stream>>>(
-            c->E, c->f[LULESH_F_SS], c->f[LULESH_F_VDOV], c->f[LULESH_F_AREALG], c->P, c->st);
+        k_constraints<<<grid_for(c->totE, kBlock), kBlock, 0, c->stream>>>(
+            int32_t(c->totE), c->f[LULESH_F_SS], c->f[LULESH_F_VDOV], c->f[LULESH_F_AREALG], c->P,
+            c->st);
         HIP_TRY(c, hipGetLastError());
     }
     if (rc) return rc;
-    if ((rc = read_state(c))) return rc;
+    if ((rc = read_states(c))) return rc;
     return c->hst->error;
 }
 
 int lulesh_hip_constraints_result(lulesh_ctx *c, double *dtcourant, double *dthydro) {
     if (!c) return LULESH_INVALID_ARGUMENT;
-    int rc = read_state(c);
+    int rc = read_states(c);
     if (rc) return rc;
     if (dtcourant) *dtcourant = bits_to_double(c->hst->dtc_bits);
     if (dthydro) *dthydro = bits_to_double(c->hst->dth_bits);
@@ -1001,106 +1276,166 @@ int lulesh_hip_constraints_result(lulesh_ctx *c, double *dtcourant, double *dthy
 }
 
 int lulesh_hip_run(lulesh_ctx *c, int32_t max_cycles, lulesh_time_state *io) {
-    if (!c || !io) return LULESH_INVALID_ARGUMENT;
-    DevState s = c->init;
+    if (!c || !io || c->nprob != 1) return LULESH_INVALID_ARGUMENT;
+    int rc = read_states(c); // keeps the node-buffer bookkeeping (cur, active)
+    if (rc) return rc;
+    DevState &s = c->hst[0];
     s.time = io->time;
     s.deltatime = io->deltatime;
     s.dtc_bits = host_dbits(io->dtcourant);
     s.dth_bits = host_dbits(io->dthydro);
     s.cycle = io->cycle;
     s.its = max_cycles;
-    s.active = 1;
     s.error = 0;
-    *c->hst = s;
-    HIP_TRY(c, hipMemcpyAsync(c->st, c->hst, sizeof s, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    // Cycles per graph: enough to amortise launch and polling, few enough
-    // that the no-op tail after the last cycle stays short.
-    const int per = c->E >= 1000000 ? 8 : 32;
-    int rc = 0;
-    if (!c->timing && (rc = build_graph(c, per))) return rc;
-    for (;;) {
-        if (c->timing) {
-            for (int k = 0; k < per && rc == 0; ++k) rc = enqueue_cycle(c);
-            if (rc) return rc;
-        } else {
-            HIP_TRY(c, hipGraphLaunch(c->graph, c->stream));
-        }
-        if ((rc = read_state(c))) return rc;
-        const DevState &h = *c->hst;
-        if (h.error || !(h.time < h.stoptime && h.cycle < h.its)) break;
-    }
-    const DevState &h = *c->hst;
-    io->time = h.time;
-    io->deltatime = h.deltatime;
-    io->dtcourant = bits_to_double(h.dtc_bits);
-    io->dthydro = bits_to_double(h.dth_bits);
-    io->cycle = h.cycle;
-    io->status = h.error;
-    if (h.error) c->err = "VolumeError";
-    return h.error;
+    if ((rc = write_states(c))) return rc;
+    rc = run_loop(c);
+    to_time_state(c->hst[0], io);
+    return rc;
 }
 
+int lulesh_hip_batch_run(lulesh_ctx *c, int32_t max_cycles, lulesh_time_state *states) {
+    if (!c) return LULESH_INVALID_ARGUMENT;
+    int rc = read_states(c);
+    if (rc) return rc;
+    for (int pi = 0; pi < c->nprob; ++pi) c->hst[pi].its = max_cycles;
+    if ((rc = write_states(c))) return rc;
+    rc = run_loop(c);
+    if (states)
+        for (int pi = 0; pi < c->nprob; ++pi) to_time_state(c->hst[pi], &states[pi]);
+    return rc;
+}
+
+double lulesh_hip_last_run_ms(const lulesh_ctx *c) { return c ? c->last_run_ms : 0.0; }
+
 int lulesh_hip_download(lulesh_ctx *c, int id, void *dst, size_t count) {
-    if (!c || !dst || id < 0 || id >= LULESH_F_COUNT) return LULESH_INVALID_ARGUMENT;
-    const size_t n = is_node_field(id) ? size_t(c->N) : size_t(c->E);
-    if (count > n) return LULESH_INVALID_ARGUMENT;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (c->precision == LULESH_PRECISION_MIXED && is_grad_field(id)) {
-        double *tmp = nullptr;
-        HIP_TRY(c, hipMalloc(&tmp, n * sizeof(double)));
-        k_widen<float><<<grid_for(int64_t(n), 256), 256, 0, c->stream>>>(int32_t(n),
-                                                                         c->g32[grad_slot(id)], tmp);
-        hipError_t e = hipMemcpyAsync(dst, tmp, count * sizeof(double), hipMemcpyDeviceToHost,
-                                      c->stream);
-        (void)hipStreamSynchronize(c->stream);
-        (void)hipFree(tmp);
-        if (e != hipSuccess) return hip_fail(c, e, "download");
-        return 0;
-    }
-    HIP_TRY(c, hipMemcpyAsync(dst, c->f[id], count * sizeof(double), hipMemcpyDeviceToHost,
-                              c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    return 0;
+    return field_copy(c, 0, id, dst, count, true);
 }
 
 int lulesh_hip_upload(lulesh_ctx *c, int id, const void *src, size_t count) {
-    if (!c || !src || id < 0 || id >= LULESH_F_COUNT) return LULESH_INVALID_ARGUMENT;
-    const size_t n = is_node_field(id) ? size_t(c->N) : size_t(c->E);
-    if (count > n) return LULESH_INVALID_ARGUMENT;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (c->precision == LULESH_PRECISION_MIXED && is_grad_field(id)) {
-        double *tmp = nullptr;
-        HIP_TRY(c, hipMalloc(&tmp, n * sizeof(double)));
-        hipError_t e = hipMemcpy(tmp, src, count * sizeof(double), hipMemcpyHostToDevice);
-        if (e == hipSuccess) {
-            k_narrow<float><<<grid_for(int64_t(count), 256), 256, 0, c->stream>>>(
-                int32_t(count), tmp, c->g32[grad_slot(id)]);
-            e = hipStreamSynchronize(c->stream);
-        }
-        (void)hipFree(tmp);
-        if (e != hipSuccess) return hip_fail(c, e, "upload");
-        return 0;
-    }
-    HIP_TRY(c, hipMemcpy(c->f[id], src, count * sizeof(double), hipMemcpyHostToDevice));
-    return 0;
+    return field_copy(c, 0, id, const_cast<void *>(src), count, false);
+}
+
+int lulesh_hip_batch_download(lulesh_ctx *c, int32_t prob, int id, void *dst, size_t count) {
+    return field_copy(c, prob, id, dst, count, true);
 }
 
 int lulesh_hip_download_view(lulesh_ctx *c, lulesh_domain_view *dst) {
-    if (!c || !dst) return LULESH_INVALID_ARGUMENT;
+    if (!c || !dst || c->nprob != 1) return LULESH_INVALID_ARGUMENT;
     for (int id = 0; id < LULESH_F_COUNT; ++id) {
         double *p = view_field(dst, id);
         if (!p) continue;
-        int rc = lulesh_hip_download(c, id, p, is_node_field(id) ? size_t(c->N) : size_t(c->E));
+        int rc = lulesh_hip_download(c, id, p, is_node_field(id) ? size_t(c->totN)
+                                                                 : size_t(c->totE));
         if (rc) return rc;
     }
-    int rc = read_state(c);
+    int rc = read_states(c);
     if (rc) return rc;
     dst->time = c->hst->time;
     dst->deltatime = c->hst->deltatime;
     dst->dtcourant = bits_to_double(c->hst->dtc_bits);
     dst->dthydro = bits_to_double(c->hst->dth_bits);
     dst->cycle = c->hst->cycle;
+    return 0;
+}
+
+int lulesh_hip_set_tuning(lulesh_ctx *c, const lulesh_tuning *t) {
+    if (!c || !t) return LULESH_INVALID_ARGUMENT;
+    c->tune = *t;
+    return configure(c);
+}
+
+int lulesh_hip_get_tuning(const lulesh_ctx *c, lulesh_tuning *t) {
+    if (!c || !t) return LULESH_INVALID_ARGUMENT;
+    *t = c->tune;
+    return 0;
+}
+
+int lulesh_hip_autotune(lulesh_ctx *c, int32_t cycles, lulesh_tuning *best,
+                        double *best_ms_per_cycle) {
+    if (!c || cycles < 2) return LULESH_INVALID_ARGUMENT;
+    // Snapshot every state buffer; each candidate runs `cycles` cycles from
+    // the snapshot and the state is restored afterwards.
+    Snapshot snap;
+    for (int id = 0; id < LULESH_F_COUNT; ++id)
+        if (c->f[id] && !is_state_node_field(id))
+            snap.regions.push_back({c->f[id], size_t(is_node_field(id) ? c->totN : c->totE) * 8});
+    for (int k = 0; k < 6; ++k) {
+        if (c->g32[k]) snap.regions.push_back({c->g32[k], size_t(c->totE) * 4});
+        snap.regions.push_back({c->nb.b[0][k], size_t(c->totN) * 8});
+        if (c->nb.b[1][k] != c->nb.b[0][k])
+            snap.regions.push_back({c->nb.b[1][k], size_t(c->totN) * 8});
+    }
+    snap.regions.push_back({c->st, sizeof(DevState) * size_t(c->nprob)});
+    for (auto &r : snap.regions) {
+        void *q = nullptr;
+        if (hipMalloc(&q, r.second) != hipSuccess) {
+            for (void *p : snap.copies) (void)hipFree(p);
+            return hip_fail(c, hipErrorOutOfMemory, "autotune snapshot");
+        }
+        (void)hipMemcpyAsync(q, r.first, r.second, hipMemcpyDeviceToDevice, c->stream);
+        snap.copies.push_back(q);
+    }
+    const bool timing0 = c->timing;
+    c->timing = false;
+    int rc = read_states(c);
+    std::vector<DevState> st0(c->hst, c->hst + c->nprob);
+    auto restore = [&]() -> int {
+        for (size_t k = 0; k < snap.regions.size(); ++k)
+            HIP_TRY(c, hipMemcpyAsync(snap.regions[k].first, snap.copies[k],
+                                      snap.regions[k].second, hipMemcpyDeviceToDevice,
+                                      c->stream));
+        return read_states(c);
+    };
+    auto trial = [&](const lulesh_tuning &t, double *ms) -> int {
+        c->tune = t;
+        int r = configure(c);
+        if (r) return r;
+        if ((r = restore())) return r;
+        for (int pi = 0; pi < c->nprob; ++pi) c->hst[pi].its = st0[size_t(pi)].cycle + cycles;
+        if ((r = write_states(c))) return r;
+        if ((r = build_graph(c))) return r; // graph build outside the timing
+        r = run_loop(c);
+        *ms = c->last_run_ms / cycles;
+        return r;
+    };
+    lulesh_tuning cur = c->tune;
+    cur.zplanes = auto_zplanes(c);
+    double cur_ms = 1e300;
+    if (rc == 0) rc = trial(cur, &cur_ms);
+    if (rc == 0) cur = c->tune;
+    // coordinate descent over the launch parameters
+    const int zcand[] = {2, 3, 4, 6, 8, 12, 16, 24, 32};
+    const int bcand[] = {64, 128, 256};
+    const int gcand[] = {4, 8, 16, 32};
+    for (int pass = 0; pass < 4 && rc == 0; ++pass) {
+        const int *cand = pass == 0 ? zcand : pass == 3 ? gcand : bcand;
+        const int ncand = pass == 0 ? 9 : pass == 3 ? 4 : 3;
+        for (int k = 0; k < ncand && rc == 0; ++k) {
+            lulesh_tuning t = cur;
+            int32_t *field = pass == 0   ? &t.zplanes
+                             : pass == 1 ? &t.block_elem
+                             : pass == 2 ? &t.block_material
+                                         : &t.cycles_per_graph;
+            if (*field == cand[k]) continue;
+            *field = cand[k];
+            double ms = 0;
+            rc = trial(t, &ms);
+            if (rc == 0 && ms < cur_ms) {
+                cur = c->tune;
+                cur_ms = ms;
+            }
+        }
+    }
+    c->tune = cur;
+    const int rc2 = configure(c);
+    const int rc3 = restore();
+    for (void *p : snap.copies) (void)hipFree(p);
+    c->timing = timing0;
+    if (rc) return rc;
+    if (rc2) return rc2;
+    if (rc3) return rc3;
+    if (best) *best = cur;
+    if (best_ms_per_cycle) *best_ms_per_cycle = cur_ms;
     return 0;
 }
 
@@ -1129,6 +1464,8 @@ int lulesh_hip_kernel_times(lulesh_ctx *c, const char **names, double *avg_ms,
 
 int64_t lulesh_hip_device_bytes(const lulesh_ctx *c) { return c ? c->bytes : 0; }
 
+int32_t lulesh_hip_is_structured(const lulesh_ctx *c) { return c && c->structured ? 1 : 0; }
+
 const char *lulesh_hip_last_error(const lulesh_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
 void lulesh_hip_destroy(lulesh_ctx *c) {
@@ -1136,9 +1473,12 @@ void lulesh_hip_destroy(lulesh_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     for (void *p : c->allocs) (void)hipFree(p);
+    for (void *p : {(void *)c->d_pd, (void *)c->d_segs, (void *)c->d_zBlockProb,
+                    (void *)c->d_eBlockProb, (void *)c->d_mBlockSeg})
+        if (p) (void)hipFree(p);
     if (c->hst) (void)hipHostFree(c->hst);
-    if (c->ev0) (void)hipEventDestroy(c->ev0);
-    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t e : {c->ev0, c->ev1, c->run0, c->run1})
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -41,23 +41,55 @@ struct NodeForceOut { // optional copies of fx.. / xdd.. (Domain fidelity)
     double *fx, *fy, *fz, *xdd, *ydd, *zdd;
 };
 
+// One problem of a context (a context holds one Domain or a batch of
+// independent Domains stored back to back).
+struct ProbDesc {
+    int32_t s, E, N;
+    int32_t elemOff, nodeOff; // offsets into the concatenated element / node arrays
+    int32_t zplanes, tiles;   // z-march: node planes per workgroup, node tiles per edge
+    int32_t zBlock0, eBlock0; // first workgroup of this problem in the z-march / element grids
+};
+
 template <typename HG, bool STORE_F>
 __global__ __launch_bounds__(kZmarchThreads) void k_nodal_zmarch(
-    int32_t s, int32_t planes_per_block, NodeIn in, NodeOut out, NodeForceOut fo,
-    const double *__restrict__ p, const double *__restrict__ q, const double *__restrict__ volo,
-    const double *__restrict__ v, const double *__restrict__ ss,
-    const double *__restrict__ elemMass, const double *__restrict__ nodalMass, double hgcoef,
-    double u_cut, DevState *st) {
+    const ProbDesc *__restrict__ pds, const int32_t *__restrict__ blockProb, NodeBufs nb,
+    NodeForceOut fo0, const double *__restrict__ p0,
+    const double *__restrict__ q0, const double *__restrict__ volo0,
+    const double *__restrict__ v0, const double *__restrict__ ss0,
+    const double *__restrict__ elemMass0, const double *__restrict__ nodalMass0, double hgcoef,
+    double u_cut, DevState *states) {
+    const int prob = blockProb[blockIdx.x];
+    DevState *st = states + prob;
     if (!st->active) return;
+    const ProbDesc pd = pds[prob];
+    const int s = pd.s, planes_per_block = pd.zplanes;
+    const int eo = pd.elemOff, no = pd.nodeOff;
+    const int cur = st->cur;
+    double *const *bi = nb.b[cur];
+    double *const *bo = nb.b[cur ^ 1];
+    const NodeIn in{bi[0] + no, bi[1] + no, bi[2] + no, bi[3] + no, bi[4] + no, bi[5] + no};
+    const NodeOut out{bo[0] + no, bo[1] + no, bo[2] + no, bo[3] + no, bo[4] + no, bo[5] + no};
+    NodeForceOut fo = fo0;
+    if (STORE_F) fo = {fo0.fx + no, fo0.fy + no, fo0.fz + no, fo0.xdd + no, fo0.ydd + no,
+                       fo0.zdd + no};
+    const double *__restrict__ p = p0 + eo;
+    const double *__restrict__ q = q0 + eo;
+    const double *__restrict__ volo = volo0 + eo;
+    const double *__restrict__ v = v0 + eo;
+    const double *__restrict__ ss = ss0 + eo;
+    const double *__restrict__ elemMass = elemMass0 + eo;
+    const double *__restrict__ nodalMass = nodalMass0 + no;
+    const int lb = blockIdx.x - pd.zBlock0;
     // LDS: 4 corners x (3 stress + 3 hourglass) values per element of the patch
     __shared__ double lds[24][kZmarchThreads];
 
     const int n = s + 1;
     const int tid = threadIdx.x;
     const int jj = tid / kPatch, ii = tid % kPatch;
-    const int tilesX = (n + kTile - 1) / kTile;
-    const int J0 = (blockIdx.x / tilesX) * kTile, I0 = (blockIdx.x % tilesX) * kTile;
-    const int K0 = blockIdx.y * planes_per_block;
+    const int tilesX = pd.tiles;
+    const int tile = lb % (tilesX * tilesX);
+    const int J0 = (tile / tilesX) * kTile, I0 = (tile % tilesX) * kTile;
+    const int K0 = (lb / (tilesX * tilesX)) * planes_per_block;
     const int K1 = min(K0 + planes_per_block, n); // owned node planes [K0, K1)
     const int ej = J0 - 1 + jj, ei = I0 - 1 + ii; // this thread's element column
     const bool elemCol = (ej >= 0) && (ej < s) && (ei >= 0) && (ei < s);

@@ -3,6 +3,7 @@
 // the host Domain (lulesh_domain.h).
 #include <cmath>
 #include <new>
+#include <random>
 
 #include "domain.hpp"
 #include "lulesh_domain.h"
@@ -143,6 +144,32 @@ void lulesh_verify_report(lulesh_domain *d, double out[4]) {
 
 int32_t lulesh_region_rep(int32_t reg, int32_t numReg, int32_t cost) {
     return lulesh::RegionRep(reg, numReg, cost);
+}
+
+int lulesh_domain_scale_energy(lulesh_domain *d, double factor) {
+    if (!d || !(factor > 0)) return LULESH_INVALID_ARGUMENT;
+    Domain &dom = d->dom;
+    const double einit = dom.e(0) * factor;
+    dom.e(0) = einit;
+    dom.deltatime() = (0.5 * std::cbrt(dom.volo(0))) / std::sqrt(2.0 * einit);
+    return 0;
+}
+
+int lulesh_batch_params(uint64_t seed, int32_t nprob, int32_t smin, int32_t smax,
+                        int32_t *sizes, int32_t *regions, double *factors) {
+    if (nprob < 0 || smin < 1 || smax < smin || !sizes || !regions || !factors)
+        return LULESH_INVALID_ARGUMENT;
+    // Raw engine outputs mapped by hand so the draws do not depend on the
+    // standard library's distribution algorithms.
+    std::mt19937_64 gen(seed);
+    const uint64_t span = uint64_t(smax - smin + 1);
+    static const int32_t kRegions[3] = {11, 16, 21};
+    for (int32_t p = 0; p < nprob; ++p) {
+        sizes[p] = smin + int32_t(gen() % span);
+        regions[p] = kRegions[gen() % 3];
+        factors[p] = 0.5 + 1.5 * double(gen() >> 11) * 0x1.0p-53;
+    }
+    return 0;
 }
 
 } // extern "C"

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "liblulesh_amd.so")
 
 OK, VOLUME_ERROR, QSTOP_ERROR, INVALID_ARGUMENT = 0, -1, -2, -3
 FP64, MIXED = 0, 1
+OPT_NODE_FORCES, OPT_GENERIC = 0x100, 0x200
 PHASE_NODAL, PHASE_ELEMENTS, PHASE_MATERIAL, PHASE_CONSTRAINTS = 0, 1, 2, 3
 
 FIELDS = ("x y z xd yd zd xdd ydd zdd fx fy fz nodalMass e p q ql qq v volo delv vdov "
@@ -52,17 +53,12 @@ class TimeState(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("block_force", ctypes.c_int32), ("block_elem", ctypes.c_int32),
+    _fields_ = [("zplanes", ctypes.c_int32), ("block_elem", ctypes.c_int32),
                 ("block_material", ctypes.c_int32), ("cycles_per_graph", ctypes.c_int32)]
 
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
-# Every symbol the headers declare (checked by the CPU test suite).
-EXPORTS = """lulesh_hip_create lulesh_hip_leapfrog lulesh_hip_run lulesh_hip_phase
-lulesh_hip_constraints_result lulesh_hip_download lulesh_hip_upload lulesh_hip_download_view
-lulesh_hip_set_timing lulesh_hip_kernel_times lulesh_hip_device_bytes lulesh_hip_last_error
-lulesh_hip_destroy lulesh_domain_create lulesh_domain_destroy lulesh_domain_get_view
-lulesh_domain_set_time_state lulesh_domain_get_time_state lulesh_time_increment
-lulesh_lagrange_leapfrog lulesh_verify_report lulesh_region_rep""".split()
 
 _lib = None
 
@@ -106,25 +102,24 @@ def lib():
         fn = getattr(L, name)
         fn.restype = res
         fn.argtypes = args
-    for name in ("lulesh_hip_batch_create", "lulesh_hip_batch_run", "lulesh_hip_batch_download",
-                 "lulesh_hip_batch_last_ms", "lulesh_hip_batch_last_error",
-                 "lulesh_hip_batch_destroy"):
-        if hasattr(L, name):
-            EXPORTS.append(name) if name not in EXPORTS else None
-    if hasattr(L, "lulesh_hip_batch_create"):
-        L.lulesh_hip_batch_create.restype = ctypes.c_int
-        L.lulesh_hip_batch_create.argtypes = [ctypes.POINTER(ctypes.POINTER(DomainView)), i32,
-                                              ctypes.POINTER(Tuning), ctypes.POINTER(vp)]
-        L.lulesh_hip_batch_run.restype = ctypes.c_int
-        L.lulesh_hip_batch_run.argtypes = [vp, i32, ctypes.POINTER(TimeState)]
-        L.lulesh_hip_batch_download.restype = ctypes.c_int
-        L.lulesh_hip_batch_download.argtypes = [vp, i32, ctypes.c_int, vp, ctypes.c_size_t]
-        L.lulesh_hip_batch_last_ms.restype = ctypes.c_double
-        L.lulesh_hip_batch_last_ms.argtypes = [vp]
-        L.lulesh_hip_batch_last_error.restype = ctypes.c_char_p
-        L.lulesh_hip_batch_last_error.argtypes = [vp]
-        L.lulesh_hip_batch_destroy.restype = None
-        L.lulesh_hip_batch_destroy.argtypes = [vp]
+    more = {
+        "lulesh_hip_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.POINTER(DomainView)),
+                                                   i32, ctypes.c_int, ctypes.POINTER(vp)]),
+        "lulesh_hip_batch_run": (ctypes.c_int, [vp, i32, ctypes.POINTER(TimeState)]),
+        "lulesh_hip_batch_download": (ctypes.c_int, [vp, i32, ctypes.c_int, vp, ctypes.c_size_t]),
+        "lulesh_hip_num_problems": (i32, [vp]),
+        "lulesh_hip_last_run_ms": (dbl, [vp]),
+        "lulesh_hip_is_structured": (i32, [vp]),
+        "lulesh_hip_set_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
+        "lulesh_hip_get_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
+        "lulesh_hip_autotune": (ctypes.c_int, [vp, i32, ctypes.POINTER(Tuning), _D]),
+        "lulesh_domain_scale_energy": (ctypes.c_int, [vp, dbl]),
+        "lulesh_batch_params": (ctypes.c_int, [ctypes.c_uint64, i32, i32, i32, _I, _I, _D]),
+    }
+    for name, (res, args) in more.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
     _lib = L
     return L
 
@@ -204,12 +199,12 @@ class Domain:
 class Device:
     """A device context (lulesh_hip_create) holding one Domain in HBM."""
 
-    def __init__(self, dom, precision=FP64):
+    def __init__(self, dom, precision=FP64, opts=0):
         self.L = lib()
         self.dom = dom
         self.E, self.N = dom.E, dom.N
         h = ctypes.c_void_p()
-        rc = self.L.lulesh_hip_create(ctypes.byref(dom.view), precision, ctypes.byref(h))
+        rc = self.L.lulesh_hip_create(ctypes.byref(dom.view), precision | opts, ctypes.byref(h))
         if rc != 0:
             raise LuleshError(rc, "lulesh_hip_create")
         self.h = h
@@ -274,12 +269,113 @@ class Device:
     def device_bytes(self):
         return self.L.lulesh_hip_device_bytes(self.h)
 
+    def structured(self):
+        return bool(self.L.lulesh_hip_is_structured(self.h))
 
-def run_device(s, its, nr=11, balance=1, cost=1, precision=FP64):
+    def last_run_ms(self):
+        return self.L.lulesh_hip_last_run_ms(self.h)
+
+    def tuning(self):
+        t = Tuning()
+        self.L.lulesh_hip_get_tuning(self.h, ctypes.byref(t))
+        return t
+
+    def set_tuning(self, t):
+        self._check(self.L.lulesh_hip_set_tuning(self.h, ctypes.byref(t)), "set_tuning")
+
+    def autotune(self, cycles):
+        t = Tuning()
+        ms = ctypes.c_double()
+        self._check(self.L.lulesh_hip_autotune(self.h, cycles, ctypes.byref(t), ctypes.byref(ms)),
+                    "autotune")
+        return t, ms.value
+
+
+def batch_params(seed, nprob, smin=20, smax=80):
+    """Seeded ensemble parameters (sizes, region counts, energy factors)."""
+    L = lib()
+    s = np.zeros(nprob, np.int32)
+    r = np.zeros(nprob, np.int32)
+    f = np.zeros(nprob, np.float64)
+    rc = L.lulesh_batch_params(seed, nprob, smin, smax, s.ctypes.data_as(_I),
+                               r.ctypes.data_as(_I), f.ctypes.data_as(_D))
+    if rc != 0:
+        raise LuleshError(rc, "lulesh_batch_params")
+    return s, r, f
+
+
+def make_domain(s, nr=11, factor=1.0, balance=1, cost=1):
+    dom = Domain(s, nr, balance, cost)
+    if factor != 1.0:
+        rc = dom.L.lulesh_domain_scale_energy(dom.h, factor)
+        if rc != 0:
+            raise LuleshError(rc, "scale_energy")
+        dom.refresh_view()
+    return dom
+
+
+class Batch:
+    """Many Domains in one device context (lulesh_hip_batch_create)."""
+
+    def __init__(self, doms, precision=FP64, opts=0):
+        self.L = lib()
+        self.doms = doms
+        arr = (ctypes.POINTER(DomainView) * len(doms))(*[ctypes.pointer(d.view) for d in doms])
+        h = ctypes.c_void_p()
+        rc = self.L.lulesh_hip_batch_create(arr, len(doms), precision | opts, ctypes.byref(h))
+        if rc != 0:
+            raise LuleshError(rc, "lulesh_hip_batch_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.lulesh_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def run(self, max_cycles):
+        states = (TimeState * len(self.doms))()
+        rc = self.L.lulesh_hip_batch_run(self.h, max_cycles, states)
+        return rc, list(states)
+
+    def download(self, prob, name):
+        d = self.doms[prob]
+        n = d.N if name in NODE_FIELDS else d.E
+        out = np.empty(n, np.float64)
+        rc = self.L.lulesh_hip_batch_download(self.h, prob, FIELD_ID[name], out.ctypes.data, n)
+        if rc != 0:
+            raise LuleshError(rc, "batch_download")
+        return out
+
+    def last_run_ms(self):
+        return self.L.lulesh_hip_last_run_ms(self.h)
+
+    def tuning(self):
+        t = Tuning()
+        self.L.lulesh_hip_get_tuning(self.h, ctypes.byref(t))
+        return t
+
+    def set_tuning(self, t):
+        rc = self.L.lulesh_hip_set_tuning(self.h, ctypes.byref(t))
+        if rc != 0:
+            raise LuleshError(rc, "set_tuning")
+
+    def autotune(self, cycles):
+        t = Tuning()
+        ms = ctypes.c_double()
+        rc = self.L.lulesh_hip_autotune(self.h, cycles, ctypes.byref(t), ctypes.byref(ms))
+        if rc != 0:
+            raise LuleshError(rc, "autotune")
+        return t, ms.value
+
+
+def run_device(s, its, nr=11, balance=1, cost=1, precision=FP64, opts=0):
     """Build a Domain, run `its` cycles (or to stoptime) on the device with
     the device-side time loop; returns (domain, device, time_state)."""
     dom = Domain(s, nr, balance, cost)
-    dev = Device(dom, precision)
+    dev = Device(dom, precision, opts)
     ts = dom.time_state()
     rc = dev.run(its, ts)
     if rc != 0:

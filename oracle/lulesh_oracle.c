@@ -744,6 +744,12 @@ ora_domain *ora_create(int32_t nx, int32_t nr, int32_t balance, int32_t cost) {
     return d;
 }
 
+void ora_scale_energy(ora_domain *d, double factor) {
+    Real_t einit = d->e[0] * factor;
+    d->e[0] = einit;
+    d->deltatime = (.5 * cbrt(d->volo[0])) / sqrt(2.0 * einit);
+}
+
 void ora_destroy(ora_domain *d) {
     if (!d) return;
     Real_t *rf[] = {d->x,    d->y,     d->z,       d->xd,       d->yd,      d->zd,

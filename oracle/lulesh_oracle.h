@@ -44,6 +44,10 @@ int32_t ora_reg_elem_size(ora_domain *d, int32_t r);
 int32_t *ora_reg_elem_list(ora_domain *d, int32_t r);
 int32_t ora_reg_rep(ora_domain *d, int32_t r);
 
+/* Ensemble member: scale the deposit of element 0 by `factor` and recompute
+ * the initial time step as the constructor does (lulesh-init.cc:172-181). */
+void ora_scale_energy(ora_domain *d, double factor);
+
 /* Threads used by the OpenMP loops (1 = serial).  Results do not depend on it. */
 void ora_set_threads(int32_t nthreads);
 int32_t ora_max_threads(void);

@@ -44,6 +44,7 @@ def lib():
         L.ora_reg_rep.restype = c_int
         L.ora_reg_rep.argtypes = [vp, c_int]
         L.ora_set_threads.argtypes = [c_int]
+        L.ora_scale_energy.argtypes = [vp, c_dbl]
         L.ora_max_threads.restype = c_int
         L.ora_time_increment.argtypes = [vp]
         for fn in ("ora_lagrange_nodal", "ora_lagrange_elements",
@@ -91,6 +92,9 @@ class OracleDomain:
         if not self.h:
             raise ValueError("ora_create failed")
         self.s = s
+
+    def scale_energy(self, factor):
+        self.L.ora_scale_energy(self.h, float(factor))
 
     def close(self):
         if self.h:

@@ -1,0 +1,128 @@
+"""GPU checks of the batched ensemble (many Domains in one context) and of
+the autotuner: every problem of a batch must evolve exactly as it does when
+run alone, with its own time step, time and cycle count."""
+import numpy as np
+import pytest
+
+from oracle_lib import OracleDomain
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ["x", "y", "z", "xd", "yd", "zd", "e", "p", "q", "v", "ss", "vdov", "arealg"]
+
+
+def banner(msg):
+    print(msg, flush=True)
+
+
+def single_run(pyl, s, nr, factor, its, precision=0):
+    dom = pyl.make_domain(s, nr, factor)
+    dev = pyl.Device(dom, precision)
+    ts = dom.time_state()
+    assert dev.run(its, ts) == 0
+    return dev, ts
+
+
+def test_batch_params_deterministic(pyl):
+    s1, r1, f1 = pyl.batch_params(0, 64)
+    s2, r2, f2 = pyl.batch_params(0, 64)
+    assert (s1 == s2).all() and (r1 == r2).all() and (f1 == f2).all()
+    assert s1.min() >= 20 and s1.max() <= 80
+    assert set(r1) <= {11, 16, 21} and f1.min() >= 0.5 and f1.max() < 2.0
+
+
+def test_batch_equals_individual_runs(pyl):
+    banner("batch of 5 vs individual runs")
+    params = [(5, 11, 1.0), (9, 16, 0.6), (12, 21, 1.7), (7, 11, 1.3), (16, 16, 0.9)]
+    its = 70
+    doms = [pyl.make_domain(s, nr, f) for s, nr, f in params]
+    batch = pyl.Batch(doms)
+    rc, states = batch.run(its)
+    assert rc == 0
+    for k, (s, nr, f) in enumerate(params):
+        dev, ts = single_run(pyl, s, nr, f, its)
+        assert states[k].cycle == ts.cycle == its
+        assert states[k].time == ts.time and states[k].deltatime == ts.deltatime
+        for name in FIELDS:
+            np.testing.assert_array_equal(batch.download(k, name), dev.download(name),
+                                          err_msg=f"problem {k} {name}")
+
+
+def test_batch_problem_matches_oracle(pyl):
+    banner("batch member vs oracle (scaled energy)")
+    doms = [pyl.make_domain(6, 11, 1.0), pyl.make_domain(10, 21, 1.8)]
+    batch = pyl.Batch(doms)
+    rc, states = batch.run(120)
+    assert rc == 0
+    ora = OracleDomain(10, 21)
+    ora.scale_energy(1.8)
+    assert ora.run(120) == 0
+    assert states[1].cycle == ora.cycle
+    e = batch.download(1, "e")
+    assert np.max(np.abs(e - ora.f64("e"))) <= 1e-9 * np.max(np.abs(ora.f64("e")))
+
+
+def test_batch_per_problem_stop_time(pyl):
+    """Small problems reach the stop time at their own cycle counts while
+    the batch continues (each with its own dt)."""
+    banner("batch to completion s=4..7")
+    params = [(4, 11, 0.5), (5, 11, 2.0), (6, 16, 1.0), (7, 21, 1.4)]
+    doms = [pyl.make_domain(*p) for p in params]
+    batch = pyl.Batch(doms)
+    rc, states = batch.run(10 ** 6)
+    assert rc == 0
+    cycles = set()
+    for k, p in enumerate(params):
+        dev, ts = single_run(pyl, *p, 10 ** 6)
+        assert states[k].cycle == ts.cycle
+        assert abs(states[k].time - 1e-2) < 1e-12
+        np.testing.assert_array_equal(batch.download(k, "e"), dev.download("e"))
+        cycles.add(ts.cycle)
+    assert len(cycles) > 1
+
+
+def test_autotune_keeps_state(pyl):
+    """The autotuner restores the state it started from: a run after tuning
+    equals a run without it, bit for bit."""
+    banner("autotune")
+    params = [(10, 11, 1.0), (14, 16, 1.2)]
+    a = pyl.Batch([pyl.make_domain(*p) for p in params])
+    rc, _ = a.run(20)
+    assert rc == 0
+    t, ms = a.autotune(6)
+    assert t.zplanes >= 1 and t.block_elem in (64, 128, 256) and ms > 0
+    rc, sa = a.run(50)
+    b = pyl.Batch([pyl.make_domain(*p) for p in params])
+    rc2, sb = b.run(50)
+    assert rc == rc2 == 0
+    for k in range(2):
+        assert sa[k].cycle == sb[k].cycle == 50
+        np.testing.assert_array_equal(a.download(k, "e"), b.download(k, "e"))
+        np.testing.assert_array_equal(a.download(k, "x"), b.download(k, "x"))
+
+
+@pytest.mark.parametrize("tune", [(3, 64, 128, 4), (7, 128, 64, 6), (40, 256, 256, 16)])
+def test_tunings_do_not_change_results(pyl, tune):
+    banner(f"tuning {tune}")
+    dom = pyl.Domain(18)
+    dev = pyl.Device(dom)
+    t = pyl.Tuning(*tune)
+    dev.set_tuning(t)
+    ts = dom.time_state()
+    assert dev.run(45, ts) == 0
+    ref, _ = single_run(pyl, 18, 11, 1.0, 45)
+    for name in FIELDS:
+        np.testing.assert_array_equal(dev.download(name), ref.download(name), err_msg=name)
+
+
+def test_mixed_batch(pyl):
+    banner("mixed batch")
+    params = [(8, 11, 1.0), (12, 16, 1.5)]
+    batch = pyl.Batch([pyl.make_domain(*p) for p in params], precision=pyl.MIXED)
+    rc, states = batch.run(100)
+    assert rc == 0
+    for k, p in enumerate(params):
+        dev, ts = single_run(pyl, *p, 100)
+        e64 = dev.download("e")[0]
+        e32 = batch.download(k, "e")[0]
+        assert abs(e32 - e64) <= 1e-6 * abs(e64)

@@ -47,15 +47,19 @@ def test_setup_matches_oracle(pyl, s):
         np.testing.assert_array_equal(dom.field(f), ora.i32(f), err_msg=f)
 
 
-@pytest.mark.parametrize("s", [3, 6])
-def test_phases_bitwise_first_cycles(pyl, s):
+PATHS = {"structured": 0, "generic": 0x200}
+
+
+@pytest.mark.parametrize("path", ["structured", "generic"])
+@pytest.mark.parametrize("s", [3, 6, 17])
+def test_phases_bitwise_first_cycles(pyl, s, path):
     """Cycles 1-3 from the Sedov start, phase by phase.  Velocities are zero
     before the first nodal phase, so no cbrt-dependent term reaches the state
     until cycle 3's hourglass force: cycles 1-2 are bitwise."""
-    banner(f"phases s={s}")
+    banner(f"phases s={s} {path}")
     ora = OracleDomain(s)
     dom = pyl.Domain(s)
-    dev = pyl.Device(dom)
+    dev = pyl.Device(dom, opts=pyl.OPT_NODE_FORCES | PATHS[path])
     for cyc in range(2):
         ora.time_increment()
         dt = ora.scalar("deltatime")
@@ -77,18 +81,19 @@ def test_phases_bitwise_first_cycles(pyl, s):
         assert dtc == ora.scalar("dtcourant") and dth == ora.scalar("dthydro")
 
 
-@pytest.mark.parametrize("s", [4, 8])
-def test_phases_from_oracle_state(pyl, s):
+@pytest.mark.parametrize("path", ["structured", "generic"])
+@pytest.mark.parametrize("s", [4, 8, 16, 31])
+def test_phases_from_oracle_state(pyl, s, path):
     """Phase-level parity on a developed state (after 30 oracle cycles):
     the oracle's state is uploaded, each device phase is compared with the
     oracle phase from the same inputs.  Exact except the hourglass cbrt:
     tolerance 1e-13 relative to the field's max (a 1-ulp coefficient
     difference moves a force by ~2e-16 of itself)."""
-    banner(f"developed-state phases s={s}")
+    banner(f"developed-state phases s={s} {path}")
     ora = OracleDomain(s)
     ora.run(30)
     dom = pyl.Domain(s)
-    dev = pyl.Device(dom)
+    dev = pyl.Device(dom, opts=pyl.OPT_NODE_FORCES | PATHS[path])
     copy_state(pyl, ora, dev)
     ora.time_increment()
     dt = ora.scalar("deltatime")
@@ -114,13 +119,14 @@ def test_phases_from_oracle_state(pyl, s):
     assert (dtc2, dth2) == (dtc, dth)
 
 
-@pytest.mark.parametrize("s,its", [(5, 50), (10, 100), (30, 100)])
-def test_device_loop_vs_oracle(pyl, s, its):
+@pytest.mark.parametrize("path", ["structured", "generic"])
+@pytest.mark.parametrize("s,its", [(5, 50), (10, 100), (30, 100), (16, 300)])
+def test_device_loop_vs_oracle(pyl, s, its, path):
     """Device time loop (lulesh_hip_run) against the oracle's main loop."""
-    banner(f"device loop s={s} its={its}")
+    banner(f"device loop s={s} its={its} {path}")
     ora = OracleDomain(s)
     assert ora.run(its) == 0
-    dom, dev, ts = pyl.run_device(s, its)
+    dom, dev, ts = pyl.run_device(s, its, opts=PATHS[path])
     assert ts.cycle == ora.cycle == its
     assert abs(ts.time - ora.scalar("time")) <= 1e-12 * ora.scalar("time")
     for f in STATE:
@@ -134,18 +140,47 @@ def test_device_loop_vs_oracle(pyl, s, its):
 
 def test_host_loop_matches_device_loop(pyl):
     """The reference loop shape (host TimeIncrement + one leapfrog call per
-    cycle) and the device-side loop give identical states."""
+    cycle) and the device-side loop give identical states; odd cycle counts
+    exercise the node double buffer's parity."""
     banner("host loop vs device loop s=8")
-    dom_a, dev_a, ts_a = pyl.run_device(8, 40)
+    dom_a, dev_a, ts_a = pyl.run_device(8, 41)
     dom_b = pyl.Domain(8)
     dev_b = pyl.Device(dom_b)
     L = pyl.lib()
-    for _ in range(40):
+    for _ in range(41):
         dom_b.time_increment()
         assert L.lulesh_lagrange_leapfrog(dom_b.h, dev_b.h) == 0
     dev_b.download_view()
     for f in STATE:
         np.testing.assert_array_equal(dom_a.field(f), dom_b.field(f), err_msg=f)
+
+
+def test_structured_equals_generic_and_mixed_loops(pyl):
+    """The structured z-march path and the indexed path compute the same
+    arithmetic in the same order: identical states after 100 cycles; a run
+    interleaving host-driven cycles (odd count) with device loops too."""
+    banner("structured == generic s=12")
+    a, _, ta = pyl.run_device(12, 100)
+    b, _, tb = pyl.run_device(12, 100, opts=pyl.OPT_GENERIC)
+    for f in STATE:
+        np.testing.assert_array_equal(a.field(f), b.field(f), err_msg=f)
+    dom = pyl.Domain(12)
+    dev = pyl.Device(dom)
+    L = pyl.lib()
+    for _ in range(3):  # three host-driven cycles flip the node buffer parity
+        dom.time_increment()
+        assert L.lulesh_lagrange_leapfrog(dom.h, dev.h) == 0
+    ts = dom.time_state()
+    assert dev.run(50, ts) == 0 and ts.cycle == 50
+    dom.set_time_state(ts)
+    for _ in range(3):
+        dom.time_increment()
+        assert L.lulesh_lagrange_leapfrog(dom.h, dev.h) == 0
+    ts = dom.time_state()
+    assert dev.run(100, ts) == 0 and ts.cycle == 100
+    dev.download_view()
+    for f in STATE:
+        np.testing.assert_array_equal(dom.field(f), a.field(f), err_msg=f)
 
 
 def test_s30_to_completion(pyl):
