@@ -424,8 +424,17 @@ __device__ __forceinline__ double ss_floor_sqrt(double ssc) {
     return (ssc <= .1111111e-36) ? .3333333e-18 : sqrt_(ssc);
 }
 
+// Division by the reference density.  x / 1.0 == x exactly, so with the
+// reference's refdens = 1 (lulesh-init.cc:36) the division is skipped
+// without changing a bit.
+template <bool RHO1>
+__device__ __forceinline__ double per_rho0(double x, double rho0) {
+    return RHO1 ? x : x / rho0;
+}
+
 // EvalEOSAllInOneTask for one element (lulesh.cc:1692-1871) followed by the
 // sound speed of CalcSoundSpeedForElemsAndSaveTask (lulesh.cc:1885-1896).
+template <bool RHO1>
 __device__ __forceinline__ void eos(double e_old, double delvc, double p_old, double q_old,
                                     double qq_old, double ql_old, double vnewc, const Params &P,
                                     double &e_out, double &p_out, double &q_out,
@@ -452,7 +461,7 @@ __device__ __forceinline__ void eos(double e_old, double delvc, double p_old, do
             q_new = 0.;
         } else {
             const double ssc =
-                ss_floor_sqrt((pbvc * e_new + vhalf * vhalf * bvc * pHalfStep) / rho0);
+                ss_floor_sqrt(per_rho0<RHO1>(pbvc * e_new + vhalf * vhalf * bvc * pHalfStep, rho0));
             q_new = (ssc * ql_old + qq_old);
         }
         e_new = e_new + 0.5 * delvc * (3.0 * (p_old + q_old) - 4.0 * (pHalfStep + q_new));
@@ -471,7 +480,7 @@ __device__ __forceinline__ void eos(double e_old, double delvc, double p_old, do
             q_tilde = 0.;
         } else {
             const double ssc =
-                ss_floor_sqrt((pbvc * e_new + vnewc * vnewc * bvc * p_new) / rho0);
+                ss_floor_sqrt(per_rho0<RHO1>(pbvc * e_new + vnewc * vnewc * bvc * p_new, rho0));
             q_tilde = (ssc * ql_old + qq_old);
         }
         e_new = e_new - (7.0 * (p_old + q_old) - 8.0 * (pHalfStep + q_new) + (p_new + q_tilde)) *
@@ -481,11 +490,11 @@ __device__ __forceinline__ void eos(double e_old, double delvc, double p_old, do
     }
     p_new = pressure(bvc, pbvc, e_new, compression, vnewc, P);
     if (delvc <= 0.) {
-        const double ssc = ss_floor_sqrt((pbvc * e_new + vnewc * vnewc * bvc * p_new) / rho0);
+        const double ssc = ss_floor_sqrt(per_rho0<RHO1>(pbvc * e_new + vnewc * vnewc * bvc * p_new, rho0));
         q_new = (ssc * ql_old + qq_old);
         if (fabs_(q_new) < P.q_cut) q_new = 0.;
     }
-    ss_out = ss_floor_sqrt((pbvc * e_new + vnewc * vnewc * bvc * p_new) / rho0);
+    ss_out = ss_floor_sqrt(per_rho0<RHO1>(pbvc * e_new + vnewc * vnewc * bvc * p_new, rho0));
     e_out = e_new;
     p_out = p_new;
     q_out = q_new;

@@ -400,7 +400,7 @@ __device__ __forceinline__ void block_min_atomic(double c, double h, DevState *s
 // then the element's Courant and hydro constraints (1938-1974) folded into
 // the block minimum.
 // ---------------------------------------------------------------------------
-template <typename QG>
+template <typename QG, bool RHO1>
 __global__ __launch_bounds__(kBlock) void k_material(
     const MatSeg *__restrict__ segs, const int32_t *__restrict__ blockSeg,
     const int32_t *__restrict__ list, ElemFields f, const QG *__restrict__ delv_xi,
@@ -453,7 +453,7 @@ __global__ __launch_bounds__(kBlock) void k_material(
             opaque(qqi);
             opaque(qli);
             opaque(vi);
-            eos(ei, di, pi, qi, qqi, qli, vi, P, en, pn, qn, ssn);
+            eos<RHO1>(ei, di, pi, qi, qqi, qli, vi, P, en, pn, qn, ssn);
         }
         f.ss[i] = ssn;
         f.p[i] = pn;
@@ -767,7 +767,8 @@ int enqueue_material(lulesh_ctx *c) {
                         c->f[LULESH_F_E], c->f[LULESH_F_P], c->f[LULESH_F_Q], c->f[LULESH_F_QL],
                         c->f[LULESH_F_QQ], c->f[LULESH_F_SS]};
     return launch(c, T_MATERIAL, [&] {
-        k_material<R><<<c->mBlocks, c->tune.block_material, 0, c->stream>>>(
+        auto kern = c->P.refdens == 1.0 ? k_material<R, true> : k_material<R, false>;
+        kern<<<c->mBlocks, c->tune.block_material, 0, c->stream>>>(
             c->d_segs, c->d_mBlockSeg, c->matList, ef, grad_ptr<R>(c, 0), grad_ptr<R>(c, 1),
             grad_ptr<R>(c, 2), grad_ptr<R>(c, 3), grad_ptr<R>(c, 4), grad_ptr<R>(c, 5), c->P,
             c->st);
