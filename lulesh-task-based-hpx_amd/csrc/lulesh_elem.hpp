@@ -224,6 +224,12 @@ __device__ __forceinline__ constexpr double gamma_hg(int mode, int node) {
 // CalcElemFBHourglassForce (527-562).  T = double (FP64 form) or float
 // (mixed form: coordinates are shifted to node 0 in FP64 before rounding so
 // that the FP32 differences keep their significant bits).
+//
+// The modes are streamed one at a time: column i1 of hourgam is built, its
+// three projections hxx are formed, and each corner force accumulates
+// hourgam[i][i1] * hxx[i1].  Accumulating i1 = 0..3 in order is exactly the
+// reference's left-to-right four-term sum (lulesh.cc:539-540), so the
+// result is bit-identical while only one column of hourgam is live.
 template <typename T>
 __device__ __forceinline__ void hourglass_force(const T x[8], const T y[8], const T z[8],
                                                 const T xd[8], const T yd[8], const T zd[8],
@@ -231,7 +237,7 @@ __device__ __forceinline__ void hourglass_force(const T x[8], const T y[8], cons
     T dvdx[8], dvdy[8], dvdz[8];
     volume_derivative<T>(dvdx, dvdy, dvdz, x, y, z);
     const T volinv = T(1.0) / determ;
-    T hourgam[8][4];
+    T hmx[4], hmy[4], hmz[4];
 #pragma unroll
     for (int i1 = 0; i1 < 4; ++i1) {
         T hx = x[0] * T(gamma_hg(i1, 0)), hy = y[0] * T(gamma_hg(i1, 0)),
@@ -242,27 +248,31 @@ __device__ __forceinline__ void hourglass_force(const T x[8], const T y[8], cons
             hy = hy + y[j] * T(gamma_hg(i1, j));
             hz = hz + z[j] * T(gamma_hg(i1, j));
         }
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-            hourgam[j][i1] =
-                T(gamma_hg(i1, j)) - volinv * (dvdx[j] * hx + dvdy[j] * hy + dvdz[j] * hz);
+        hmx[i1] = hx;
+        hmy[i1] = hy;
+        hmz[i1] = hz;
     }
     const T *vel[3] = {xd, yd, zd};
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        T hxx[4];
+    for (int i1 = 0; i1 < 4; ++i1) {
+        T col[8];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            T s = hourgam[0][i] * vel[c][0];
+        for (int j = 0; j < 8; ++j)
+            col[j] = T(gamma_hg(i1, j)) -
+                     volinv * (dvdx[j] * hmx[i1] + dvdy[j] * hmy[i1] + dvdz[j] * hmz[i1]);
 #pragma unroll
-            for (int j = 1; j < 8; ++j) s = s + hourgam[j][i] * vel[c][j];
-            hxx[i] = s;
+        for (int c = 0; c < 3; ++c) {
+            T h = col[0] * vel[c][0];
+#pragma unroll
+            for (int j = 1; j < 8; ++j) h = h + col[j] * vel[c][j];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) hgf[c][i] = (i1 == 0) ? col[i] * h : hgf[c][i] + col[i] * h;
         }
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-            hgf[c][i] = coefficient * (hourgam[i][0] * hxx[0] + hourgam[i][1] * hxx[1] +
-                                       hourgam[i][2] * hxx[2] + hourgam[i][3] * hxx[3]);
     }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) hgf[c][i] = coefficient * hgf[c][i];
 }
 
 // AreaFace, lulesh.cc:678-692

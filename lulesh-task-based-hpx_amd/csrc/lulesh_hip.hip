@@ -400,9 +400,10 @@ __device__ __forceinline__ void block_min_atomic(double c, double h, DevState *s
 // then the element's Courant and hydro constraints (1938-1974) folded into
 // the block minimum.
 // ---------------------------------------------------------------------------
-template <typename QG, bool RHO1>
+template <typename QG, bool RHO1, bool STRUCT>
 __global__ __launch_bounds__(kBlock) void k_material(
-    const MatSeg *__restrict__ segs, const int32_t *__restrict__ blockSeg,
+    const ProbDesc *__restrict__ pds, const MatSeg *__restrict__ segs,
+    const int32_t *__restrict__ blockSeg,
     const int32_t *__restrict__ list, ElemFields f, const QG *__restrict__ delv_xi,
     const QG *__restrict__ delv_eta, const QG *__restrict__ delv_zeta,
     const QG *__restrict__ delx_xi, const QG *__restrict__ delx_eta,
@@ -417,19 +418,24 @@ __global__ __launch_bounds__(kBlock) void k_material(
         const int32_t bc = f.elemBC[i];
         const double sxi = double(delv_xi[i]), seta = double(delv_eta[i]),
                      szeta = double(delv_zeta[i]);
+        // On the structured cube the neighbour the switch reads is the
+        // adjacent element (checked at create), so no index array is loaded.
+        int ps = 0;
+        if (STRUCT) ps = pds[sg.prob].s;
         auto nbr = [&](const QG *a, int32_t bits, int32_t comm, int32_t symm, const int32_t *l,
-                       double self) {
-            return (bits == comm || bits == 0) ? double(a[l[i]]) : (bits == symm ? self : 0.0);
+                       int step, double self) {
+            if (bits == comm || bits == 0) return double(a[STRUCT ? i + step : l[i]]);
+            return bits == symm ? self : 0.0;
         };
         const double phixi = mono_phi(
-            sxi, nbr(delv_xi, bc & XI_M, XI_M_COMM, XI_M_SYMM, f.lxim, sxi),
-            nbr(delv_xi, bc & XI_P, XI_P_COMM, XI_P_SYMM, f.lxip, sxi), P);
+            sxi, nbr(delv_xi, bc & XI_M, XI_M_COMM, XI_M_SYMM, f.lxim, -1, sxi),
+            nbr(delv_xi, bc & XI_P, XI_P_COMM, XI_P_SYMM, f.lxip, 1, sxi), P);
         const double phieta = mono_phi(
-            seta, nbr(delv_eta, bc & ETA_M, ETA_M_COMM, ETA_M_SYMM, f.letam, seta),
-            nbr(delv_eta, bc & ETA_P, ETA_P_COMM, ETA_P_SYMM, f.letap, seta), P);
+            seta, nbr(delv_eta, bc & ETA_M, ETA_M_COMM, ETA_M_SYMM, f.letam, -ps, seta),
+            nbr(delv_eta, bc & ETA_P, ETA_P_COMM, ETA_P_SYMM, f.letap, ps, seta), P);
         const double phizeta = mono_phi(
-            szeta, nbr(delv_zeta, bc & ZETA_M, ZETA_M_COMM, ZETA_M_SYMM, f.lzetam, szeta),
-            nbr(delv_zeta, bc & ZETA_P, ZETA_P_COMM, ZETA_P_SYMM, f.lzetap, szeta), P);
+            szeta, nbr(delv_zeta, bc & ZETA_M, ZETA_M_COMM, ZETA_M_SYMM, f.lzetam, -ps * ps, szeta),
+            nbr(delv_zeta, bc & ZETA_P, ZETA_P_COMM, ZETA_P_SYMM, f.lzetap, ps * ps, szeta), P);
         const double dv[3] = {sxi, seta, szeta};
         const double dx[3] = {double(delx_xi[i]), double(delx_eta[i]), double(delx_zeta[i])};
         const double vdov = f.vdov[i], vnew = f.vnew[i];
@@ -767,9 +773,11 @@ int enqueue_material(lulesh_ctx *c) {
                         c->f[LULESH_F_E], c->f[LULESH_F_P], c->f[LULESH_F_Q], c->f[LULESH_F_QL],
                         c->f[LULESH_F_QQ], c->f[LULESH_F_SS]};
     return launch(c, T_MATERIAL, [&] {
-        auto kern = c->P.refdens == 1.0 ? k_material<R, true> : k_material<R, false>;
+        const bool rho1 = c->P.refdens == 1.0;
+        auto kern = c->structured ? (rho1 ? k_material<R, true, true> : k_material<R, false, true>)
+                                  : (rho1 ? k_material<R, true, false> : k_material<R, false, false>);
         kern<<<c->mBlocks, c->tune.block_material, 0, c->stream>>>(
-            c->d_segs, c->d_mBlockSeg, c->matList, ef, grad_ptr<R>(c, 0), grad_ptr<R>(c, 1),
+            c->d_pd, c->d_segs, c->d_mBlockSeg, c->matList, ef, grad_ptr<R>(c, 0), grad_ptr<R>(c, 1),
             grad_ptr<R>(c, 2), grad_ptr<R>(c, 3), grad_ptr<R>(c, 4), grad_ptr<R>(c, 5), c->P,
             c->st);
     });
@@ -904,6 +912,18 @@ bool is_structured_cube(const lulesh_domain_view *v) {
         const int32_t g = v->nodelist[c];
         if (v->nodeElemCornerList[fill[size_t(g)]++] != int32_t(c)) return false;
     }
+    // Face neighbours: wherever the BC switch of lulesh.cc:1460-1597 reads a
+    // neighbour (bits 0 or *_COMM), it must be the adjacent element.
+    const int32_t *nb[6] = {v->lxim, v->lxip, v->letam, v->letap, v->lzetam, v->lzetap};
+    const int32_t masks[6] = {XI_M, XI_P, ETA_M, ETA_P, ZETA_M, ZETA_P};
+    const int32_t comms[6] = {XI_M_COMM, XI_P_COMM, ETA_M_COMM, ETA_P_COMM, ZETA_M_COMM,
+                              ZETA_P_COMM};
+    const int64_t step[6] = {-1, 1, -s, s, -s * s, s * s};
+    for (int64_t e = 0; e < E; ++e)
+        for (int d = 0; d < 6; ++d) {
+            const int32_t bits = v->elemBC[e] & masks[d];
+                        if ((bits == 0 || bits == comms[d]) && nb[d][e] != e + step[d]) return false;
+        }
     if (!v->symmX || !v->symmY || !v->symmZ) return false;
     const int64_t P = n * n;
     for (int axis = 0; axis < 3; ++axis) {

@@ -30,6 +30,7 @@ namespace lulesh_dev {
 constexpr int kPatch = 16;            // element patch edge (threads per tile edge)
 constexpr int kTile = kPatch - 1;     // node tile edge owned by a workgroup
 constexpr int kZmarchThreads = kPatch * kPatch;
+constexpr int kPatchNodes = (kPatch + 1) * (kPatch + 1); // node columns of a patch
 
 struct NodeIn {
     const double *x, *y, *z, *xd, *yd, *zd;
@@ -80,8 +81,11 @@ __global__ __launch_bounds__(kZmarchThreads) void k_nodal_zmarch(
     const double *__restrict__ elemMass = elemMass0 + eo;
     const double *__restrict__ nodalMass = nodalMass0 + no;
     const int lb = blockIdx.x - pd.zBlock0;
-    // LDS: 4 corners x (3 stress + 3 hourglass) values per element of the patch
+    // LDS: 4 corners x (3 stress + 3 hourglass) values per element of the
+    // patch, and the 17x17 patch nodes of the two node planes an element
+    // plane touches (a ring of two planes: x, y, z, xd, yd, zd each).
     __shared__ double lds[24][kZmarchThreads];
+    __shared__ double pn[2][6][kPatchNodes];
 
     const int n = s + 1;
     const int tid = threadIdx.x;
@@ -103,8 +107,10 @@ __global__ __launch_bounds__(kZmarchThreads) void k_nodal_zmarch(
 
     double px = 0.0, py = 0.0, pz = 0.0; // partial sums of the node in the current plane
 
+    const int own = (jj + 1) * (kPatch + 1) + (ii + 1); // node (J, I) in the patch node array
     auto finish_node = [&](int K) {
         const int g = (K * n + J) * n + I;
+        const int slot = K & 1;
         const double mass = nodalMass[g];
         double ax = px / mass, ay = py / mass, az = pz / mass;
         if (STORE_F) {
@@ -120,18 +126,36 @@ __global__ __launch_bounds__(kZmarchThreads) void k_nodal_zmarch(
             fo.ydd[g] = ay;
             fo.zdd[g] = az;
         }
-        double vx = in.xd[g] + ax * dt;
+        double vx = pn[slot][3][own] + ax * dt;
         if (fabs_(vx) < u_cut) vx = 0.0;
-        double vy = in.yd[g] + ay * dt;
+        double vy = pn[slot][4][own] + ay * dt;
         if (fabs_(vy) < u_cut) vy = 0.0;
-        double vz = in.zd[g] + az * dt;
+        double vz = pn[slot][5][own] + az * dt;
         if (fabs_(vz) < u_cut) vz = 0.0;
         out.xd[g] = vx;
         out.yd[g] = vy;
         out.zd[g] = vz;
-        out.x[g] = in.x[g] + vx * dt;
-        out.y[g] = in.y[g] + vy * dt;
-        out.z[g] = in.z[g] + vz * dt;
+        out.x[g] = pn[slot][0][own] + vx * dt;
+        out.y[g] = pn[slot][1][own] + vy * dt;
+        out.z[g] = pn[slot][2][own] + vz * dt;
+    };
+    // Load node plane K of the patch into ring slot K & 1 (coalesced rows of
+    // 17 nodes; nodes outside the mesh are not touched -- no element reads them).
+    auto load_plane = [&](int K) {
+        const int slot = K & 1;
+        for (int t = tid; t < kPatchNodes; t += kZmarchThreads) {
+            const int a = t / (kPatch + 1), b = t - a * (kPatch + 1);
+            const int Jn = J0 - 1 + a, In = I0 - 1 + b;
+            if (Jn >= 0 && Jn < n && In >= 0 && In < n) {
+                const int g = (K * n + Jn) * n + In;
+                pn[slot][0][t] = in.x[g];
+                pn[slot][1][t] = in.y[g];
+                pn[slot][2][t] = in.z[g];
+                pn[slot][3][t] = in.xd[g];
+                pn[slot][4][t] = in.yd[g];
+                pn[slot][5][t] = in.zd[g];
+            }
+        }
     };
     // add stress then hourglass of one corner staged at lds[4c.. ] of patch thread t
     auto add = [&](int c, int t) {
@@ -145,42 +169,58 @@ __global__ __launch_bounds__(kZmarchThreads) void k_nodal_zmarch(
 
     const int kbeg = max(K0 - 1, 0);
     const int kend = min(K1, s); // element planes [kbeg, kend)
+    load_plane(kbeg);
     for (int k = kbeg; k < kend; ++k) {
+        // plane k is in its slot; bring in plane k+1 over plane k-1, whose
+        // last readers finished before the previous iteration's final barrier
+        load_plane(k + 1);
+        __syncthreads();
         // ---- element (k, ej, ei): stress + hourglass corner forces ----------
         double up[24]; // corners 4..7: (sx, sy, sz, hx, hy, hz) each
         if (elemCol) {
             const int e = (k * s + ej) * s + ei;
-            const int b = (k * n + ej) * n + ei;
-            const int nd[8] = {b, b + 1, b + n + 1, b + n, b + n * n, b + n * n + 1,
-                               b + n * n + n + 1, b + n * n + n};
+            // the element's nodes in the patch node arrays (reference local
+            // order, lulesh-init.cc:237-244): bottom plane k, top plane k+1
+            const int q0 = jj * (kPatch + 1) + ii;
+            const int nq[4] = {q0, q0 + 1, q0 + kPatch + 2, q0 + kPatch + 1};
+            const int sb = k & 1, st1 = sb ^ 1;
             double x[8], y[8], z[8];
 #pragma unroll
-            for (int a = 0; a < 8; ++a) {
-                x[a] = in.x[nd[a]];
-                y[a] = in.y[nd[a]];
-                z[a] = in.z[nd[a]];
+            for (int a = 0; a < 4; ++a) {
+                x[a] = pn[sb][0][nq[a]];
+                y[a] = pn[sb][1][nq[a]];
+                z[a] = pn[sb][2][nq[a]];
+                x[a + 4] = pn[st1][0][nq[a]];
+                y[a + 4] = pn[st1][1][nq[a]];
+                z[a + 4] = pn[st1][2][nq[a]];
             }
-            double sc[3][8];
             {
+                double sc[3][8];
                 const double det = shape_derivatives<false>(x, y, z, nullptr);
                 if (det == 0) st->error = kVolumeError;
                 node_normals(sc, x, y, z);
                 const double sig = -p[e] - q[e];
+                // corners 0..3 go to LDS now (its last readers passed the
+                // previous barrier); corners 4..7 wait in registers
 #pragma unroll
-                for (int a = 0; a < 8; ++a) {
-                    sc[0][a] = -(sig * sc[0][a]);
-                    sc[1][a] = -(sig * sc[1][a]);
-                    sc[2][a] = -(sig * sc[2][a]);
-                }
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        lds[6 * a + d][tid] = -(sig * sc[d][a]);
+                        up[6 * a + d] = -(sig * sc[d][a + 4]);
+                    }
             }
             const double determ = volo[e] * v[e];
             if (determ <= 0.0) st->error = kVolumeError;
             double xd[8], yd[8], zd[8];
 #pragma unroll
-            for (int a = 0; a < 8; ++a) {
-                xd[a] = in.xd[nd[a]];
-                yd[a] = in.yd[nd[a]];
-                zd[a] = in.zd[nd[a]];
+            for (int a = 0; a < 4; ++a) {
+                xd[a] = pn[sb][3][nq[a]];
+                yd[a] = pn[sb][4][nq[a]];
+                zd[a] = pn[sb][5][nq[a]];
+                xd[a + 4] = pn[st1][3][nq[a]];
+                yd[a + 4] = pn[st1][4][nq[a]];
+                zd[a + 4] = pn[st1][5][nq[a]];
             }
             double hg[3][8];
             if constexpr (sizeof(HG) == 8) {
@@ -212,9 +252,7 @@ __global__ __launch_bounds__(kZmarchThreads) void k_nodal_zmarch(
             for (int c = 0; c < 4; ++c) {
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    lds[6 * c + d][tid] = sc[d][c];
                     lds[6 * c + 3 + d][tid] = hg[d][c];
-                    up[6 * c + d] = sc[d][c + 4];
                     up[6 * c + 3 + d] = hg[d][c + 4];
                 }
             }
