@@ -44,6 +44,17 @@ struct NodeBufs {
 
 constexpr int32_t kVolumeError = -1; // lulesh.h:30
 
+// XCD-aware block order.  Workgroups are dispatched round-robin over the
+// eight XCDs of an MI355X, each with its own L2; this bijection gives every
+// XCD a contiguous range of logical blocks, so neighbouring tiles and
+// element ranges (which share node data) meet in the same L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int bid, int nblocks) {
+    const int per = nblocks / kXcds, rem = nblocks % kXcds;
+    const int xcd = bid % kXcds, idx = bid / kXcds;
+    return xcd * per + min(xcd, rem) + idx;
+}
+
 // Run constants (lulesh-init.cc:18-36) passed by value to the kernels.
 struct Params {
     double e_cut, p_cut, q_cut, v_cut, u_cut, hgcoef, monoq_max_slope, monoq_limiter_mult,

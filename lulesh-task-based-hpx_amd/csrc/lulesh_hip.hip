@@ -276,10 +276,11 @@ __global__ __launch_bounds__(kBlock) void k_elems(
     QG *__restrict__ dveta, QG *__restrict__ dvzeta, QG *__restrict__ dxxi,
     QG *__restrict__ dxeta, QG *__restrict__ dxzeta, double v_cut, double eosvmin,
     double eosvmax, DevState *states) {
-    const int prob = blockProb[blockIdx.x];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int prob = blockProb[blk];
     DevState *st = states + prob;
     const ProbDesc pd = pds[prob];
-    const int li = (blockIdx.x - pd.eBlock0) * blockDim.x + threadIdx.x;
+    const int li = (blk - pd.eBlock0) * blockDim.x + threadIdx.x;
     if (li >= pd.E || !st->active) return;
     const int i = pd.elemOff + li; // element index in the concatenated arrays
     const double deltaTime = st->deltatime;
@@ -408,10 +409,13 @@ __global__ __launch_bounds__(kBlock) void k_material(
     const QG *__restrict__ delv_eta, const QG *__restrict__ delv_zeta,
     const QG *__restrict__ delx_xi, const QG *__restrict__ delx_eta,
     const QG *__restrict__ delx_zeta, Params P, DevState *states) {
-    const MatSeg sg = segs[blockSeg[blockIdx.x]];
+    // (no XCD remap here: the costliest segments come first and must stay
+    // spread over all XCDs)
+    const int blk = blockIdx.x;
+    const MatSeg sg = segs[blockSeg[blk]];
     DevState *st = states + sg.prob;
     if (!st->active) return; // uniform across the block
-    const int t = (blockIdx.x - sg.block0) * blockDim.x + threadIdx.x;
+    const int t = (blk - sg.block0) * blockDim.x + threadIdx.x;
     double dtc = 1.7976931348623157e+308, dth = 1.7976931348623157e+308;
     if (t < sg.count) {
         const int i = list[sg.offset + t];
@@ -466,6 +470,137 @@ __global__ __launch_bounds__(kBlock) void k_material(
         f.e[i] = en;
         f.q[i] = qn;
         constraints(ssn, vdov, f.arealg[i], P, dtc, dth);
+    }
+    block_min_atomic(dtc, dth, st);
+}
+
+// ---------------------------------------------------------------------------
+// Material in element order.  Same per-element work as k_material, but the
+// `rep` evaluations of EvalEOSAllInOneTask of the 64 elements of a wave are
+// dealt out over its 64 lanes: the wave's sum of repetitions is split into
+// rounds of 64 evaluations, lane L of round r evaluates repetition
+// r*64 + L, which belongs to the element whose prefix range holds it.  Every
+// evaluation reads its inputs from LDS (so none can be folded into another)
+// and the element's last repetition writes its outputs.  A wave needs
+// ceil(sum rep / 64) rounds instead of max(rep), and the element fields are
+// read in order (coalesced) with one round trip to memory.
+// ---------------------------------------------------------------------------
+template <typename QG, bool RHO1, bool STRUCT>
+__global__ __launch_bounds__(kBlock) void k_material_ordered(
+    const ProbDesc *__restrict__ pds, const int32_t *__restrict__ blockProb,
+    const uint16_t *__restrict__ erep, ElemFields f, const QG *__restrict__ delv_xi,
+    const QG *__restrict__ delv_eta, const QG *__restrict__ delv_zeta,
+    const QG *__restrict__ delx_xi, const QG *__restrict__ delx_eta,
+    const QG *__restrict__ delx_zeta, Params P, DevState *states) {
+    // dynamic LDS, sized to the block: 7 inputs, 4 outputs, prefix sums
+    extern __shared__ double mat_lds[];
+    const int nt = blockDim.x;
+    double *sin_[7], *sout[4];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) sin_[k] = mat_lds + k * nt;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sout[k] = mat_lds + (7 + k) * nt;
+    int32_t *sincl = reinterpret_cast<int32_t *>(mat_lds + 11 * nt);
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int prob = blockProb[blk];
+    DevState *st = states + prob;
+    if (!st->active) return; // uniform across the block
+    const ProbDesc pd = pds[prob];
+    const int tid = threadIdx.x, lane = tid & 63, wbase = tid & ~63;
+    const int li = (blk - pd.mBlock0) * blockDim.x + tid;
+    const bool valid = li < pd.E;
+    const int i = pd.elemOff + (valid ? li : pd.E - 1);
+    double dtc = 1.7976931348623157e+308, dth = 1.7976931348623157e+308;
+    // all loads of the element and its six face neighbours are independent
+    const int32_t bc = f.elemBC[i];
+    const double sxi = double(delv_xi[i]), seta = double(delv_eta[i]),
+                 szeta = double(delv_zeta[i]);
+    double nb[6];
+    if (STRUCT) {
+        const int s = pd.s, lo = pd.elemOff, hi = pd.elemOff + pd.E - 1;
+        const int off[6] = {-1, 1, -s, s, -s * s, s * s};
+        const QG *arr[6] = {delv_xi, delv_xi, delv_eta, delv_eta, delv_zeta, delv_zeta};
+#pragma unroll
+        for (int d = 0; d < 6; ++d) nb[d] = double(arr[d][min(max(i + off[d], lo), hi)]);
+    } else {
+        nb[0] = double(delv_xi[f.lxim[i]]);
+        nb[1] = double(delv_xi[f.lxip[i]]);
+        nb[2] = double(delv_eta[f.letam[i]]);
+        nb[3] = double(delv_eta[f.letap[i]]);
+        nb[4] = double(delv_zeta[f.lzetam[i]]);
+        nb[5] = double(delv_zeta[f.lzetap[i]]);
+    }
+    const double dx[3] = {double(delx_xi[i]), double(delx_eta[i]), double(delx_zeta[i])};
+    const double vdov = f.vdov[i], vnew = f.vnew[i], emass = f.elemMass[i], volo = f.volo[i];
+    const double e0 = f.e[i], dv0 = f.delv[i], p0 = f.p[i], q0 = f.q[i], arealg = f.arealg[i];
+    const int rep = valid ? int(erep[i]) : 0;
+    // BC switch of lulesh.cc:1460-1597 on the loaded neighbours
+    auto sel = [&](int32_t bits, int32_t comm, int32_t symm, double nbv, double self) {
+        return (bits == comm || bits == 0) ? nbv : (bits == symm ? self : 0.0);
+    };
+    const double phixi = mono_phi(sxi, sel(bc & XI_M, XI_M_COMM, XI_M_SYMM, nb[0], sxi),
+                                  sel(bc & XI_P, XI_P_COMM, XI_P_SYMM, nb[1], sxi), P);
+    const double phieta = mono_phi(seta, sel(bc & ETA_M, ETA_M_COMM, ETA_M_SYMM, nb[2], seta),
+                                   sel(bc & ETA_P, ETA_P_COMM, ETA_P_SYMM, nb[3], seta), P);
+    const double phizeta =
+        mono_phi(szeta, sel(bc & ZETA_M, ZETA_M_COMM, ZETA_M_SYMM, nb[4], szeta),
+                 sel(bc & ZETA_P, ZETA_P_COMM, ZETA_P_SYMM, nb[5], szeta), P);
+    const double dv[3] = {sxi, seta, szeta};
+    double qlin, qquad;
+    q_terms(phixi, phieta, phizeta, vdov, dv, dx, emass, volo, vnew, P, qlin, qquad);
+    double vnewc = vnew;
+    if (P.eosvmin != 0.0 && vnewc < P.eosvmin) vnewc = P.eosvmin;
+    if (P.eosvmax != 0.0 && vnewc > P.eosvmax) vnewc = P.eosvmax;
+    if (valid) {
+        f.qq[i] = qquad;
+        f.ql[i] = qlin;
+    }
+    sin_[0][tid] = e0;
+    sin_[1][tid] = dv0;
+    sin_[2][tid] = p0;
+    sin_[3][tid] = q0;
+    sin_[4][tid] = qquad;
+    sin_[5][tid] = qlin;
+    sin_[6][tid] = vnewc;
+    // inclusive prefix sum of the repetition counts over the wave
+    int incl = rep;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+    }
+    const int total = __shfl(incl, 63, 64);
+    sincl[tid] = incl;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int w = lane; w < total; w += 64) {
+        // owner: the first lane whose inclusive sum exceeds w
+        int lo = 0, hi = 63;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (sincl[wbase + mid] > w) hi = mid;
+            else lo = mid + 1;
+        }
+        const int j = wbase + lo;
+        double en, pn, qn, ssn;
+        eos<RHO1>(sin_[0][j], sin_[1][j], sin_[2][j], sin_[3][j], sin_[4][j], sin_[5][j],
+                  sin_[6][j], P, en, pn, qn, ssn);
+        if (w == sincl[j] - 1) { // the element's last repetition
+            sout[0][j] = en;
+            sout[1][j] = pn;
+            sout[2][j] = qn;
+            sout[3][j] = ssn;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (valid) {
+        const double en = sout[0][tid], pn = sout[1][tid], qn = sout[2][tid], ssn = sout[3][tid];
+        f.ss[i] = ssn;
+        f.p[i] = pn;
+        f.e[i] = en;
+        f.q[i] = qn;
+        constraints(ssn, vdov, arealg, P, dtc, dth);
     }
     block_min_atomic(dtc, dth, st);
 }
@@ -546,6 +681,9 @@ struct lulesh_ctx {
     ProbDesc *d_pd = nullptr;
     MatSeg *d_segs = nullptr;
     int32_t *d_zBlockProb = nullptr, *d_eBlockProb = nullptr, *d_mBlockSeg = nullptr;
+    int32_t *d_oBlockProb = nullptr; // element-order material grid
+    int oBlocks = 0;
+    uint16_t *erep = nullptr;        // EOS repetitions per element
     // time states
     DevState *st = nullptr;
     DevState *hst = nullptr; // pinned mirror of all problems' states
@@ -650,7 +788,8 @@ int configure(lulesh_ctx *c) {
         t.block_material = 256;
     if (t.zplanes <= 0) t.zplanes = auto_zplanes(c);
     if (t.cycles_per_graph <= 0) t.cycles_per_graph = c->totE >= 1000000 ? 8 : 32;
-    std::vector<int32_t> zb, eb, mb;
+    if (t.material_mode != 1 || !c->erep) t.material_mode = c->erep ? 0 : 1;
+    std::vector<int32_t> zb, eb, mb, ob;
     c->hpd.clear();
     c->hsegs.clear();
     for (int pi = 0; pi < c->nprob; ++pi) {
@@ -669,6 +808,8 @@ int configure(lulesh_ctx *c) {
         zb.insert(zb.end(), size_t(nz), pi);
         d.eBlock0 = int32_t(eb.size());
         eb.insert(eb.end(), size_t(grid_for(ph.E, t.block_elem)), pi);
+        d.mBlock0 = int32_t(ob.size());
+        ob.insert(ob.end(), size_t(grid_for(ph.E, t.block_material)), pi);
         c->hpd.push_back(d);
     }
     // material segments, the costliest (largest rep) first so they start early
@@ -686,6 +827,7 @@ int configure(lulesh_ctx *c) {
     c->zBlocks = int(zb.size());
     c->eBlocks = int(eb.size());
     c->mBlocks = int(mb.size());
+    c->oBlocks = int(ob.size());
     auto put = [&](auto **dptr, const auto &vec) -> int {
         using T = std::remove_reference_t<decltype(vec[0])>;
         if (*dptr) (void)hipFree(*dptr);
@@ -700,7 +842,7 @@ int configure(lulesh_ctx *c) {
     int rc;
     if ((rc = put(&c->d_pd, c->hpd)) || (rc = put(&c->d_segs, c->hsegs)) ||
         (rc = put(&c->d_zBlockProb, zb)) || (rc = put(&c->d_eBlockProb, eb)) ||
-        (rc = put(&c->d_mBlockSeg, mb)))
+        (rc = put(&c->d_mBlockSeg, mb)) || (rc = put(&c->d_oBlockProb, ob)))
         return rc;
     if (c->graph) {
         (void)hipGraphExecDestroy(c->graph);
@@ -766,6 +908,27 @@ int enqueue_elements(lulesh_ctx *c) {
 
 template <typename R>
 int enqueue_material(lulesh_ctx *c) {
+    if (c->tune.material_mode == 0) {
+        const ElemFields ef{c->elemBC, c->lxim, c->lxip, c->letam, c->letap, c->lzetam,
+                            c->lzetap, c->f[LULESH_F_VDOV], c->f[LULESH_F_ELEMMASS],
+                            c->f[LULESH_F_VOLO], c->f[LULESH_F_VNEW], c->f[LULESH_F_DELV],
+                            c->f[LULESH_F_AREALG], c->f[LULESH_F_E], c->f[LULESH_F_P],
+                            c->f[LULESH_F_Q], c->f[LULESH_F_QL], c->f[LULESH_F_QQ],
+                            c->f[LULESH_F_SS]};
+        return launch(c, T_MATERIAL, [&] {
+            const bool rho1 = c->P.refdens == 1.0;
+            auto kern = c->structured
+                            ? (rho1 ? k_material_ordered<R, true, true>
+                                    : k_material_ordered<R, false, true>)
+                            : (rho1 ? k_material_ordered<R, true, false>
+                                    : k_material_ordered<R, false, false>);
+            const size_t lds = size_t(c->tune.block_material) * (11 * sizeof(double) + 4);
+            kern<<<c->oBlocks, c->tune.block_material, lds, c->stream>>>(
+                c->d_pd, c->d_oBlockProb, c->erep, ef, grad_ptr<R>(c, 0), grad_ptr<R>(c, 1),
+                grad_ptr<R>(c, 2), grad_ptr<R>(c, 3), grad_ptr<R>(c, 4), grad_ptr<R>(c, 5), c->P,
+                c->st);
+        });
+    }
     if (c->mBlocks == 0) return 0;
     const ElemFields ef{c->elemBC, c->lxim, c->lxip, c->letam, c->letap, c->lzetam, c->lzetap,
                         c->f[LULESH_F_VDOV], c->f[LULESH_F_ELEMMASS], c->f[LULESH_F_VOLO],
@@ -1100,6 +1263,21 @@ int create_impl(const lulesh_domain_view *const *views, int nprob, int precision
         }
         if ((rc = dalloc(c, &c->matList, E)) || (rc = upload(c, c->matList, all.data(), E)))
             return fail(rc);
+        // repetitions per element for the element-order material kernel
+        std::vector<uint16_t> rep(E, 0);
+        bool fits = true;
+        for (int pi = 0; pi < nprob; ++pi) {
+            const lulesh_domain_view *v = views[pi];
+            for (int r = 0; r < v->numReg; ++r) {
+                const int32_t rv = region_rep(r, v->numReg, v->cost);
+                fits = fits && rv >= 1 && rv <= 65535;
+                for (int32_t k = 0; k < v->regElemSize[r]; ++k)
+                    rep[size_t(v->regElemlist[r][k] + c->probs[size_t(pi)].elemOff)] =
+                        uint16_t(rv);
+            }
+        }
+        if (fits && ((rc = dalloc(c, &c->erep, E)) || (rc = upload(c, c->erep, rep.data(), E))))
+            return fail(rc);
     }
     if (!structured) {
         const lulesh_domain_view *v = views[0];
@@ -1428,15 +1606,17 @@ int lulesh_hip_autotune(lulesh_ctx *c, int32_t cycles, lulesh_tuning *best,
     const int zcand[] = {2, 3, 4, 6, 8, 12, 16, 24, 32};
     const int bcand[] = {64, 128, 256};
     const int gcand[] = {4, 8, 16, 32};
-    for (int pass = 0; pass < 4 && rc == 0; ++pass) {
-        const int *cand = pass == 0 ? zcand : pass == 3 ? gcand : bcand;
-        const int ncand = pass == 0 ? 9 : pass == 3 ? 4 : 3;
+    const int mcand[] = {0, 1};
+    for (int pass = 0; pass < 5 && rc == 0; ++pass) {
+        const int *cand = pass == 0 ? zcand : pass == 3 ? gcand : pass == 4 ? mcand : bcand;
+        const int ncand = pass == 0 ? 9 : pass == 3 ? 4 : pass == 4 ? 2 : 3;
         for (int k = 0; k < ncand && rc == 0; ++k) {
             lulesh_tuning t = cur;
             int32_t *field = pass == 0   ? &t.zplanes
                              : pass == 1 ? &t.block_elem
                              : pass == 2 ? &t.block_material
-                                         : &t.cycles_per_graph;
+                             : pass == 3 ? &t.cycles_per_graph
+                                         : &t.material_mode;
             if (*field == cand[k]) continue;
             *field = cand[k];
             double ms = 0;
@@ -1495,7 +1675,7 @@ void lulesh_hip_destroy(lulesh_ctx *c) {
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     for (void *p : c->allocs) (void)hipFree(p);
     for (void *p : {(void *)c->d_pd, (void *)c->d_segs, (void *)c->d_zBlockProb,
-                    (void *)c->d_eBlockProb, (void *)c->d_mBlockSeg})
+                    (void *)c->d_eBlockProb, (void *)c->d_mBlockSeg, (void *)c->d_oBlockProb})
         if (p) (void)hipFree(p);
     if (c->hst) (void)hipHostFree(c->hst);
     for (hipEvent_t e : {c->ev0, c->ev1, c->run0, c->run1})

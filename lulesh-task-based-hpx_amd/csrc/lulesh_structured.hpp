@@ -49,6 +49,7 @@ struct ProbDesc {
     int32_t elemOff, nodeOff; // offsets into the concatenated element / node arrays
     int32_t zplanes, tiles;   // z-march: node planes per workgroup, node tiles per edge
     int32_t zBlock0, eBlock0; // first workgroup of this problem in the z-march / element grids
+    int32_t mBlock0, pad;     // first workgroup in the element-order material grid
 };
 
 template <typename HG, bool STORE_F>
@@ -59,7 +60,8 @@ __global__ __launch_bounds__(kZmarchThreads) void k_nodal_zmarch(
     const double *__restrict__ v0, const double *__restrict__ ss0,
     const double *__restrict__ elemMass0, const double *__restrict__ nodalMass0, double hgcoef,
     double u_cut, DevState *states) {
-    const int prob = blockProb[blockIdx.x];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int prob = blockProb[blk];
     DevState *st = states + prob;
     if (!st->active) return;
     const ProbDesc pd = pds[prob];
@@ -80,7 +82,7 @@ __global__ __launch_bounds__(kZmarchThreads) void k_nodal_zmarch(
     const double *__restrict__ ss = ss0 + eo;
     const double *__restrict__ elemMass = elemMass0 + eo;
     const double *__restrict__ nodalMass = nodalMass0 + no;
-    const int lb = blockIdx.x - pd.zBlock0;
+    const int lb = blk - pd.zBlock0;
     // LDS: 4 corners x (3 stress + 3 hourglass) values per element of the
     // patch, and the 17x17 patch nodes of the two node planes an element
     // plane touches (a ring of two planes: x, y, z, xd, yd, zd each).

@@ -54,7 +54,8 @@ class TimeState(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("zplanes", ctypes.c_int32), ("block_elem", ctypes.c_int32),
-                ("block_material", ctypes.c_int32), ("cycles_per_graph", ctypes.c_int32)]
+                ("block_material", ctypes.c_int32), ("cycles_per_graph", ctypes.c_int32),
+                ("material_mode", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

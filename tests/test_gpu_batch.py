@@ -101,7 +101,8 @@ def test_autotune_keeps_state(pyl):
         np.testing.assert_array_equal(a.download(k, "x"), b.download(k, "x"))
 
 
-@pytest.mark.parametrize("tune", [(3, 64, 128, 4), (7, 128, 64, 6), (40, 256, 256, 16)])
+@pytest.mark.parametrize("tune", [(3, 64, 128, 4, 1), (7, 128, 64, 6, 0), (40, 256, 256, 16, 1),
+                                  (2, 256, 128, 8, 0)])
 def test_tunings_do_not_change_results(pyl, tune):
     banner(f"tuning {tune}")
     dom = pyl.Domain(18)
