@@ -440,9 +440,30 @@ __device__ __forceinline__ double pressure(double &bvc, double &pbvc, double e_o
     return newval;
 }
 
-// sqrt with the reference's floor, lulesh.cc:1779-1783
+// Square root of a finite x >= 2^-767: the reciprocal square root
+// estimate refined by two steps of the coupled Newton iteration for
+// (sqrt x, 1/(2 sqrt x)) (Markstein, "IA-64 and Elementary Functions", 2000).
+// The result is correctly rounded; it is the same operation sequence that
+// __builtin_sqrt expands to, without the denormal-range scaling and the
+// zero/infinity selects, which do not apply in this range, so the two agree
+// bit for bit on it.
+__device__ __forceinline__ double sqrt_normal(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+
+// sqrt with the reference's floor, lulesh.cc:1779-1783; the square root is
+// only taken for ssc > 1.111e-37
 __device__ __forceinline__ double ss_floor_sqrt(double ssc) {
-    return (ssc <= .1111111e-36) ? .3333333e-18 : sqrt_(ssc);
+    return (ssc <= .1111111e-36) ? .3333333e-18 : sqrt_normal(ssc);
 }
 
 // Division by the reference density.  x / 1.0 == x exactly, so with the
@@ -510,12 +531,16 @@ __device__ __forceinline__ void eos(double e_old, double delvc, double p_old, do
         if (e_new < P.emin) e_new = P.emin;
     }
     p_new = pressure(bvc, pbvc, e_new, compression, vnewc, P);
+    // The last q of CalcEnergyForElems (lulesh.cc:1848-1866) and the sound
+    // speed of CalcSoundSpeedForElemsAndSaveTask (lulesh.cc:1885-1896) take
+    // the square root of the same expression of the final state.
+    const double ssc =
+        ss_floor_sqrt(per_rho0<RHO1>(pbvc * e_new + vnewc * vnewc * bvc * p_new, rho0));
     if (delvc <= 0.) {
-        const double ssc = ss_floor_sqrt(per_rho0<RHO1>(pbvc * e_new + vnewc * vnewc * bvc * p_new, rho0));
         q_new = (ssc * ql_old + qq_old);
         if (fabs_(q_new) < P.q_cut) q_new = 0.;
     }
-    ss_out = ss_floor_sqrt(per_rho0<RHO1>(pbvc * e_new + vnewc * vnewc * bvc * p_new, rho0));
+    ss_out = ssc;
     e_out = e_new;
     p_out = p_new;
     q_out = q_new;

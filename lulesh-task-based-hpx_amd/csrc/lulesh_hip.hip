@@ -485,6 +485,9 @@ __global__ __launch_bounds__(kBlock) void k_material(
 // ceil(sum rep / 64) rounds instead of max(rep), and the element fields are
 // read in order (coalesced) with one round trip to memory.
 // ---------------------------------------------------------------------------
+constexpr int kMapWords = 16; // rounds of 64 evaluations per start map
+constexpr int kMatSlots = 11; // LDS doubles per element: 7 EOS inputs, 4 outputs
+
 template <typename QG, bool RHO1, bool STRUCT>
 __global__ __launch_bounds__(kBlock) void k_material_ordered(
     const ProbDesc *__restrict__ pds, const int32_t *__restrict__ blockProb,
@@ -492,15 +495,14 @@ __global__ __launch_bounds__(kBlock) void k_material_ordered(
     const QG *__restrict__ delv_eta, const QG *__restrict__ delv_zeta,
     const QG *__restrict__ delx_xi, const QG *__restrict__ delx_eta,
     const QG *__restrict__ delx_zeta, Params P, DevState *states) {
-    // dynamic LDS, sized to the block: 7 inputs, 4 outputs, prefix sums
+    // dynamic LDS, sized to the block: per element 7 EOS inputs and 4
+    // outputs (kMatSlots doubles, one address and immediate offsets per
+    // element), then the start maps and the inclusive repetition sums
     extern __shared__ double mat_lds[];
     const int nt = blockDim.x;
-    double *sin_[7], *sout[4];
-#pragma unroll
-    for (int k = 0; k < 7; ++k) sin_[k] = mat_lds + k * nt;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) sout[k] = mat_lds + (7 + k) * nt;
-    int32_t *sincl = reinterpret_cast<int32_t *>(mat_lds + 11 * nt);
+    unsigned long long *smap =
+        reinterpret_cast<unsigned long long *>(mat_lds + kMatSlots * nt);
+    int32_t *sincl = reinterpret_cast<int32_t *>(smap + (nt >> 6) * kMapWords);
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int prob = blockProb[blk];
     DevState *st = states + prob;
@@ -555,47 +557,70 @@ __global__ __launch_bounds__(kBlock) void k_material_ordered(
         f.qq[i] = qquad;
         f.ql[i] = qlin;
     }
-    sin_[0][tid] = e0;
-    sin_[1][tid] = dv0;
-    sin_[2][tid] = p0;
-    sin_[3][tid] = q0;
-    sin_[4][tid] = qquad;
-    sin_[5][tid] = qlin;
-    sin_[6][tid] = vnewc;
-    // inclusive prefix sum of the repetition counts over the wave
+    {
+        double *me = mat_lds + kMatSlots * tid;
+        me[0] = e0;
+        me[1] = dv0;
+        me[2] = p0;
+        me[3] = q0;
+        me[4] = qquad;
+        me[5] = qlin;
+        me[6] = vnewc;
+    }
+    // exclusive prefix sum of the repetition counts over the wave
     int incl = rep;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         const int v = __shfl_up(incl, o, 64);
         if (lane >= o) incl += v;
     }
-    const int total = __shfl(incl, 63, 64);
+    const int total = __builtin_amdgcn_readlane(incl, 63);
+    const int excl = incl - rep;
     sincl[tid] = incl;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    for (int w = lane; w < total; w += 64) {
-        // owner: the first lane whose inclusive sum exceeds w
-        int lo = 0, hi = 63;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (sincl[wbase + mid] > w) hi = mid;
-            else lo = mid + 1;
+    // Evaluation w of the wave belongs to the element whose range
+    // [excl, excl + rep) holds it.  Bit b of word k of the wave's start map
+    // marks an element whose first evaluation is k*64 + b, so the owner of
+    // evaluation r*64 + L is (starts before word r) + (starts in word r at
+    // bits <= L) - 1.  The map covers kMapWords rounds at a time.
+    unsigned long long *wmap = smap + (tid >> 6) * kMapWords;
+    int started = 0; // elements whose first evaluation precedes the current round
+    for (int c0 = 0; c0 < total; c0 += 64 * kMapWords) {
+        if (lane < kMapWords) wmap[lane] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (rep > 0 && excl >= c0 && excl < c0 + 64 * kMapWords)
+            atomicOr(&wmap[(excl - c0) >> 6], 1ull << (excl & 63));
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const int nev = min(total - c0, 64 * kMapWords);
+        for (int r = 0; r * 64 < nev; ++r) {
+            const unsigned long long m = wmap[r];
+            const uint32_t mlo = __builtin_amdgcn_readfirstlane(uint32_t(m));
+            const uint32_t mhi = __builtin_amdgcn_readfirstlane(uint32_t(m >> 32));
+            const int below = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0));
+            const int self = int((lane < 32 ? mlo >> lane : mhi >> (lane - 32)) & 1u);
+            if (r * 64 + lane < nev) {
+                const int j = wbase + started + below + self - 1;
+                double en, pn, qn, ssn;
+                double *el = mat_lds + kMatSlots * j;
+                eos<RHO1>(el[0], el[1], el[2], el[3], el[4], el[5], el[6], P, en, pn, qn, ssn);
+                if (c0 + r * 64 + lane == sincl[j] - 1) { // the element's last repetition
+                    el[7] = en;
+                    el[8] = pn;
+                    el[9] = qn;
+                    el[10] = ssn;
+                }
+            }
+            started += __builtin_popcount(mlo) + __builtin_popcount(mhi);
         }
-        const int j = wbase + lo;
-        double en, pn, qn, ssn;
-        eos<RHO1>(sin_[0][j], sin_[1][j], sin_[2][j], sin_[3][j], sin_[4][j], sin_[5][j],
-                  sin_[6][j], P, en, pn, qn, ssn);
-        if (w == sincl[j] - 1) { // the element's last repetition
-            sout[0][j] = en;
-            sout[1][j] = pn;
-            sout[2][j] = qn;
-            sout[3][j] = ssn;
-        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     if (valid) {
-        const double en = sout[0][tid], pn = sout[1][tid], qn = sout[2][tid], ssn = sout[3][tid];
+        const double *me = mat_lds + kMatSlots * tid;
+        const double en = me[7], pn = me[8], qn = me[9], ssn = me[10];
         f.ss[i] = ssn;
         f.p[i] = pn;
         f.e[i] = en;
@@ -922,7 +947,8 @@ int enqueue_material(lulesh_ctx *c) {
                                     : k_material_ordered<R, false, true>)
                             : (rho1 ? k_material_ordered<R, true, false>
                                     : k_material_ordered<R, false, false>);
-            const size_t lds = size_t(c->tune.block_material) * (11 * sizeof(double) + 4);
+            const size_t lds = size_t(c->tune.block_material) * (kMatSlots * sizeof(double) + 4) +
+                               size_t(c->tune.block_material / 64) * kMapWords * 8;
             kern<<<c->oBlocks, c->tune.block_material, lds, c->stream>>>(
                 c->d_pd, c->d_oBlockProb, c->erep, ef, grad_ptr<R>(c, 0), grad_ptr<R>(c, 1),
                 grad_ptr<R>(c, 2), grad_ptr<R>(c, 3), grad_ptr<R>(c, 4), grad_ptr<R>(c, 5), c->P,
