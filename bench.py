@@ -39,7 +39,7 @@ def model_bytes_per_zone_cycle(s):
     return 816.0 + 204.0 * (s + 1) ** 3 / s ** 3
 
 
-def kernel_bytes(name, E, N, mixed=False):
+def kernel_bytes(name, E, N, mixed=False, material_mode=0):
     """Algorithmic HBM bytes of one launch of each kernel over the mesh
     (structured path): every array element counted once per launch, node
     arrays once per node."""
@@ -51,12 +51,33 @@ def kernel_bytes(name, E, N, mixed=False):
         # node x..zd in; volo v in; v vnew delv arealg vdov out; 6 gradients out
         return E * (16 + 40 + 6 * gr) + N * 48
     if name == "material":
-        # list, elemBC, 6 neighbour indices, 6 gradients, vdov elemMass volo vnew e
-        # delv p q arealg in; qq ql p e q ss out
-        return E * (4 + 4 + 24 + 6 * gr + 72 + 48)
+        if material_mode == 0:
+            # element order: repetition count (u16), elemBC, 6 gradients, vdov
+            # elemMass volo vnew e delv p q arealg in; qq ql p e q ss out
+            return E * (2 + 4 + 6 * gr + 72 + 48)
+        # region lists: list, elemBC, 6 gradients + 6 neighbour gradients
+        # (gathered), vdov elemMass volo vnew e delv p q arealg in; qq ql p e q ss out
+        return E * (4 + 4 + 12 * gr + 72 + 48)
     if name == "time_increment":
         return 128
     return 0
+
+
+TRAFFIC_FILE = "profiles/hbm_traffic_sedov_s150_fp64.md"
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` at s=150 FP64, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); None if the
+    summary is absent."""
+    path = os.path.join(ROOT, TRAFFIC_FILE)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic" and f[1] == kernel:
+            return int(f[2])
+    return None
 
 
 def cpu_baseline(s, nr, seconds, threads):
@@ -121,6 +142,22 @@ def batch_bench(pyl, nprob, its, seed, tune_cycles, precision):
     return out
 
 
+def copy_calibration(nbytes, reps):
+    """A plain device-to-device copy of `nbytes` (torch's copy kernel), run
+    `reps` times: the reference point that scales rocprofv3's FETCH_SIZE and
+    WRITE_SIZE counters to bytes moved (each copy reads and writes nbytes)."""
+    import torch
+
+    src = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps,
+                      "note": "per copy dispatch: nbytes read, nbytes written"}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,7 +172,12 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--batch-its", type=int, default=1000)
     ap.add_argument("--tune-cycles", type=int, default=6)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(1 << 30, 5)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
               "decomposition is a design sketch, not built.", file=sys.stderr)
@@ -159,21 +201,24 @@ def main():
     e0_fp64 = float(dom.field("e")[0])
     rep64 = dom.report()
 
-    # per-kernel HIP-event timings over a few more cycles
+    # Per-kernel HIP-event timings: the next `prof_cycles` cycles run with an
+    # event pair around every kernel launch on the context's stream (the
+    # kernels and tuning of the timed region, launched one by one).
     dev.set_timing(True)
     ts2 = dom.time_state()
     ts2.time, ts2.deltatime, ts2.dtcourant, ts2.dthydro, ts2.cycle = (
         ts.time, ts.deltatime, ts.dtcourant, ts.dthydro, ts.cycle)
-    prof_cycles = 8
+    prof_cycles = 32
     rc = dev.run(ts.cycle + prof_cycles, ts2)
     assert rc == 0, rc
     kt = dev.kernel_times()
     dev.set_timing(False)
+    mode = dev.tuning().material_mode
     kernels = {}
     for name, (avg_ms, n) in kt.items():
         if n == 0:
             continue
-        b = kernel_bytes(name, E, N)
+        b = kernel_bytes(name, E, N, material_mode=mode)
         per_cycle = n / prof_cycles
         kernels[name] = {"avg_ms": round(avg_ms, 5), "launches_per_cycle": per_cycle,
                          "bytes_per_launch": b / max(per_cycle, 1),
@@ -181,6 +226,7 @@ def main():
     dom_k = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches_per_cycle"])
     dk = kernels[dom_k]
     achieved = dk["bytes_per_launch"] / (dk["avg_ms"] * 1e-3) / 1e9
+    traffic = measured_traffic(dom_k) if (s, nr) == (150, 11) else None
     ms_cycle_events = sum(k["avg_ms"] * k["launches_per_cycle"] for k in kernels.values())
     dev.close()
 
@@ -234,7 +280,8 @@ def main():
                    "s": s, "regions": nr, "elements": E},
         "roofline": {"bound": "hbm", "kernel": dom_k, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": TRAFFIC_FILE if traffic else None},
         "model_bytes_per_zone_cycle": round(bsz, 1),
         "model_hbm_frac": round(bsz * zcps / (HBM_PEAK_GBS * 1e9), 4),
         "kernels": kernels,

@@ -52,8 +52,11 @@ struct ProbDesc {
     int32_t mBlock0, pad;     // first workgroup in the element-order material grid
 };
 
+// Two waves per SIMD: the FP64 form needs about 260 VGPRs with the register
+// prefetch of the next node plane; holding it to 256 costs a handful of
+// spilled values and keeps two workgroups per CU.
 template <typename HG, bool STORE_F>
-__global__ __launch_bounds__(kZmarchThreads) void k_nodal_zmarch(
+__global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
     const ProbDesc *__restrict__ pds, const int32_t *__restrict__ blockProb, NodeBufs nb,
     NodeForceOut fo0, const double *__restrict__ p0,
     const double *__restrict__ q0, const double *__restrict__ volo0,
@@ -141,22 +144,65 @@ __global__ __launch_bounds__(kZmarchThreads) void k_nodal_zmarch(
         out.y[g] = pn[slot][1][own] + vy * dt;
         out.z[g] = pn[slot][2][own] + vz * dt;
     };
-    // Load node plane K of the patch into ring slot K & 1 (coalesced rows of
-    // 17 nodes; nodes outside the mesh are not touched -- no element reads them).
-    auto load_plane = [&](int K) {
-        const int slot = K & 1;
-        for (int t = tid; t < kPatchNodes; t += kZmarchThreads) {
-            const int a = t / (kPatch + 1), b = t - a * (kPatch + 1);
-            const int Jn = J0 - 1 + a, In = I0 - 1 + b;
-            if (Jn >= 0 && Jn < n && In >= 0 && In < n) {
-                const int g = (K * n + Jn) * n + In;
-                pn[slot][0][t] = in.x[g];
-                pn[slot][1][t] = in.y[g];
-                pn[slot][2][t] = in.z[g];
-                pn[slot][3][t] = in.xd[g];
-                pn[slot][4][t] = in.yd[g];
-                pn[slot][5][t] = in.zd[g];
+    // Node plane K of the patch goes into ring slot K & 1 in two steps:
+    // fetch_plane issues the global loads into registers (coalesced rows of
+    // 17 nodes; nodes outside the mesh are not touched -- no element reads
+    // them) and store_plane writes them to LDS one phase later, so the load
+    // latency is hidden behind the node work of the current plane.
+    // Patch node t = tid + 256 m, m = 0, 1.
+    int pg[2];
+    bool pin[2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        const int t = tid + m * kZmarchThreads;
+        const int a = t / (kPatch + 1), b = t - a * (kPatch + 1);
+        const int Jn = J0 - 1 + a, In = I0 - 1 + b;
+        pin[m] = t < kPatchNodes && Jn >= 0 && Jn < n && In >= 0 && In < n;
+        pg[m] = Jn * n + In;
+    }
+    double rn[2][6];
+    // (every path assigns the registers, so their old values die at the
+    // store and do not stay live through the element work)
+    auto fetch_plane = [&](int K, bool go) {
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            if (go && pin[m]) {
+                const int g = K * n * n + pg[m];
+                rn[m][0] = in.x[g];
+                rn[m][1] = in.y[g];
+                rn[m][2] = in.z[g];
+                rn[m][3] = in.xd[g];
+                rn[m][4] = in.yd[g];
+                rn[m][5] = in.zd[g];
+            } else {
+#pragma unroll
+                for (int f = 0; f < 6; ++f) rn[m][f] = 0.0;
             }
+        }
+    };
+    auto store_plane = [&](int K) {
+        const int slot = K & 1;
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+            if (pin[m]) {
+                const int t = tid + m * kZmarchThreads;
+#pragma unroll
+                for (int f = 0; f < 6; ++f) pn[slot][f][t] = rn[m][f];
+            }
+    };
+    // The element scalars needed first are fetched early as well:
+    // p, q, volo, v of element (k, ej, ei).
+    double ev[4];
+    auto fetch_elem = [&](int k, bool go) {
+        if (go && elemCol) {
+            const int e = (k * s + ej) * s + ei;
+            ev[0] = p[e];
+            ev[1] = q[e];
+            ev[2] = volo[e];
+            ev[3] = v[e];
+        } else {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) ev[f] = 0.0;
         }
     };
     // add stress then hourglass of one corner staged at lds[4c.. ] of patch thread t
@@ -171,16 +217,26 @@ __global__ __launch_bounds__(kZmarchThreads) void k_nodal_zmarch(
 
     const int kbeg = max(K0 - 1, 0);
     const int kend = min(K1, s); // element planes [kbeg, kend)
-    load_plane(kbeg);
+    fetch_plane(kbeg, true);
+    store_plane(kbeg);
+    fetch_plane(kbeg + 1, kbeg < kend);
+    fetch_elem(kbeg, kbeg < kend);
     for (int k = kbeg; k < kend; ++k) {
-        // plane k is in its slot; bring in plane k+1 over plane k-1, whose
-        // last readers finished before the previous iteration's final barrier
-        load_plane(k + 1);
+        // plane k is in its slot; plane k+1 (fetched during the previous
+        // step) goes over plane k-1, whose last readers (finish_node) passed
+        // the previous step's third barrier.  The previous step's last phase
+        // only read the corner staging area, which is written again after
+        // this barrier.
+        store_plane(k + 1);
         __syncthreads();
         // ---- element (k, ej, ei): stress + hourglass corner forces ----------
         double up[24]; // corners 4..7: (sx, sy, sz, hx, hy, hz) each
         if (elemCol) {
+            // sigxx = sigyy = sigzz = -p - q (lulesh.cc:629-633) and the
+            // hourglass determinant volo*v (lulesh.cc:1098)
             const int e = (k * s + ej) * s + ei;
+            const double sig = -ev[0] - ev[1];
+            const double determ = ev[2] * ev[3];
             // the element's nodes in the patch node arrays (reference local
             // order, lulesh-init.cc:237-244): bottom plane k, top plane k+1
             const int q0 = jj * (kPatch + 1) + ii;
@@ -201,7 +257,7 @@ __global__ __launch_bounds__(kZmarchThreads) void k_nodal_zmarch(
                 const double det = shape_derivatives<false>(x, y, z, nullptr);
                 if (det == 0) st->error = kVolumeError;
                 node_normals(sc, x, y, z);
-                const double sig = -p[e] - q[e];
+
                 // corners 0..3 go to LDS now (its last readers passed the
                 // previous barrier); corners 4..7 wait in registers
 #pragma unroll
@@ -212,7 +268,6 @@ __global__ __launch_bounds__(kZmarchThreads) void k_nodal_zmarch(
                         up[6 * a + d] = -(sig * sc[d][a + 4]);
                     }
             }
-            const double determ = volo[e] * v[e];
             if (determ <= 0.0) st->error = kVolumeError;
             double xd[8], yd[8], zd[8];
 #pragma unroll
@@ -262,6 +317,10 @@ __global__ __launch_bounds__(kZmarchThreads) void k_nodal_zmarch(
 #pragma unroll
             for (int c = 0; c < 24; ++c) up[c] = 0.0;
         }
+        __builtin_amdgcn_sched_barrier(0); // keep the prefetch out of the element work
+        // the next step's plane and element scalars
+        fetch_plane(k + 2, k + 1 < kend);
+        fetch_elem(k + 1, k + 1 < kend);
         __syncthreads();
         // ---- node plane k ends with corners 2,3,1,0 ----------------------------
         if (ownsNode && k >= K0) {
@@ -286,7 +345,6 @@ __global__ __launch_bounds__(kZmarchThreads) void k_nodal_zmarch(
             if (v10) add(1, tid + kPatch);    // corner 5 of element (J, I-1)
             if (v11) add(0, tid + kPatch + 1); // corner 4 of element (J, I)
         }
-        __syncthreads();
     }
     // The top node plane (K = s) has no element plane above it.
     if (ownsNode && K1 == n && kend == s && s >= K0) finish_node(s);
