@@ -367,8 +367,20 @@ struct MatSeg {
     int32_t prob, rep, offset, count, block0;
 };
 
-// Minimum over a block of two doubles, then one atomic per block.
-__device__ __forceinline__ void block_min_atomic(double c, double h, DevState *st) {
+// The Courant / hydro minima of a cycle (lulesh.cc:1938-1974, 2204-2239) are
+// reduced in two levels.  Each workgroup takes the minimum of its elements
+// and merges it with an atomic min into one of kMinSlots slots of its problem
+// (one 128-byte line each, so the atomics of different workgroups go to
+// different addresses instead of queueing on one), and k_reduce_minima then
+// folds the slots into the problem's DevState and re-arms them.  Minima are
+// exact, so the grouping does not change the result.  The values are
+// positive doubles, whose bit patterns order like the values.
+constexpr int kMinSlots = 256;
+constexpr int kSlotU64 = 16;
+constexpr unsigned long long kMinInit = 0x4415AF1D78B58C40ull; // 1.0e+20
+
+__device__ __forceinline__ void block_min_atomic(double c, double h,
+                                                 unsigned long long *prob_slots) {
     __shared__ double sc[kBlock / 64], sh[kBlock / 64];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -388,8 +400,43 @@ __device__ __forceinline__ void block_min_atomic(double c, double h, DevState *s
             c = (sc[w] < c) ? sc[w] : c;
             h = (sh[w] < h) ? sh[w] : h;
         }
-        atomicMin(&st->dtc_bits, dbits(c));
-        atomicMin(&st->dth_bits, dbits(h));
+        unsigned long long *sl = prob_slots + size_t(blockIdx.x % kMinSlots) * kSlotU64;
+        atomicMin(sl, dbits(c));
+        atomicMin(sl + 1, dbits(h));
+    }
+}
+
+// One workgroup of kMinSlots threads per problem: dtcourant / dthydro of the
+// cycle = minimum over the slots, which are reset to 1e20 for the next cycle.
+__global__ __launch_bounds__(kMinSlots) void k_reduce_minima(DevState *states,
+                                                             unsigned long long *slots) {
+    DevState *st = states + blockIdx.x;
+    if (!st->active) return;
+    unsigned long long *sl = slots + (size_t(blockIdx.x) * kMinSlots + threadIdx.x) * kSlotU64;
+    unsigned long long c = sl[0], h = sl[1];
+    sl[0] = kMinInit;
+    sl[1] = kMinInit;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long oc = __shfl_xor(c, off, 64);
+        const unsigned long long oh = __shfl_xor(h, off, 64);
+        c = oc < c ? oc : c;
+        h = oh < h ? oh : h;
+    }
+    __shared__ unsigned long long rc[kMinSlots / 64], rh[kMinSlots / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        rc[wave] = c;
+        rh[wave] = h;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kMinSlots / 64; ++w) {
+            c = rc[w] < c ? rc[w] : c;
+            h = rh[w] < h ? rh[w] : h;
+        }
+        st->dtc_bits = c;
+        st->dth_bits = h;
     }
 }
 
@@ -408,7 +455,8 @@ __global__ __launch_bounds__(kBlock) void k_material(
     const int32_t *__restrict__ list, ElemFields f, const QG *__restrict__ delv_xi,
     const QG *__restrict__ delv_eta, const QG *__restrict__ delv_zeta,
     const QG *__restrict__ delx_xi, const QG *__restrict__ delx_eta,
-    const QG *__restrict__ delx_zeta, Params P, DevState *states) {
+    const QG *__restrict__ delx_zeta, Params P, DevState *states,
+    unsigned long long *__restrict__ minslots) {
     // (no XCD remap here: the costliest segments come first and must stay
     // spread over all XCDs)
     const int blk = blockIdx.x;
@@ -471,7 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_material(
         f.q[i] = qn;
         constraints(ssn, vdov, f.arealg[i], P, dtc, dth);
     }
-    block_min_atomic(dtc, dth, st);
+    block_min_atomic(dtc, dth, minslots + size_t(sg.prob) * kMinSlots * kSlotU64);
 }
 
 // ---------------------------------------------------------------------------
@@ -494,7 +542,8 @@ __global__ __launch_bounds__(kBlock) void k_material_ordered(
     const uint16_t *__restrict__ erep, ElemFields f, const QG *__restrict__ delv_xi,
     const QG *__restrict__ delv_eta, const QG *__restrict__ delv_zeta,
     const QG *__restrict__ delx_xi, const QG *__restrict__ delx_eta,
-    const QG *__restrict__ delx_zeta, Params P, DevState *states) {
+    const QG *__restrict__ delx_zeta, Params P, DevState *states,
+    unsigned long long *__restrict__ minslots) {
     // dynamic LDS, sized to the block: per element 7 EOS inputs and 4
     // outputs (kMatSlots doubles, one address and immediate offsets per
     // element), then the start maps and the inclusive repetition sums
@@ -627,7 +676,7 @@ __global__ __launch_bounds__(kBlock) void k_material_ordered(
         f.q[i] = qn;
         constraints(ssn, vdov, arealg, P, dtc, dth);
     }
-    block_min_atomic(dtc, dth, st);
+    block_min_atomic(dtc, dth, minslots + size_t(prob) * kMinSlots * kSlotU64);
 }
 
 // Stand-alone constraint pass over one problem's elements
@@ -636,11 +685,12 @@ __global__ __launch_bounds__(kBlock) void k_material_ordered(
 __global__ __launch_bounds__(kBlock) void k_constraints(int32_t E, const double *__restrict__ ss,
                                                         const double *__restrict__ vdov,
                                                         const double *__restrict__ arealg,
-                                                        Params P, DevState *st) {
+                                                        Params P,
+                                                        unsigned long long *minslots) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double dtc = 1.7976931348623157e+308, dth = 1.7976931348623157e+308;
     if (i < E) constraints(ss[i], vdov[i], arealg[i], P, dtc, dth);
-    block_min_atomic(dtc, dth, st);
+    block_min_atomic(dtc, dth, minslots);
 }
 
 template <typename T>
@@ -709,6 +759,7 @@ struct lulesh_ctx {
     int32_t *d_oBlockProb = nullptr; // element-order material grid
     int oBlocks = 0;
     uint16_t *erep = nullptr;        // EOS repetitions per element
+    unsigned long long *minslots = nullptr; // constraint minima slots, kMinSlots per problem
     // time states
     DevState *st = nullptr;
     DevState *hst = nullptr; // pinned mirror of all problems' states
@@ -785,7 +836,7 @@ int launch(lulesh_ctx *c, int timer_id, F &&fn) {
     return 0;
 }
 
-enum { T_TIME = 0, T_NODAL, T_NODES, T_ELEMS, T_MATERIAL, T_COUNT };
+enum { T_TIME = 0, T_NODAL, T_NODES, T_ELEMS, T_MATERIAL, T_MINIMA, T_COUNT };
 
 // Node planes per z-march workgroup when the tuning leaves it open: halve the
 // z extent until the whole launch has about four workgroups per CU.
@@ -952,7 +1003,7 @@ int enqueue_material(lulesh_ctx *c) {
             kern<<<c->oBlocks, c->tune.block_material, lds, c->stream>>>(
                 c->d_pd, c->d_oBlockProb, c->erep, ef, grad_ptr<R>(c, 0), grad_ptr<R>(c, 1),
                 grad_ptr<R>(c, 2), grad_ptr<R>(c, 3), grad_ptr<R>(c, 4), grad_ptr<R>(c, 5), c->P,
-                c->st);
+                c->st, c->minslots);
         });
     }
     if (c->mBlocks == 0) return 0;
@@ -968,8 +1019,21 @@ int enqueue_material(lulesh_ctx *c) {
         kern<<<c->mBlocks, c->tune.block_material, 0, c->stream>>>(
             c->d_pd, c->d_segs, c->d_mBlockSeg, c->matList, ef, grad_ptr<R>(c, 0), grad_ptr<R>(c, 1),
             grad_ptr<R>(c, 2), grad_ptr<R>(c, 3), grad_ptr<R>(c, 4), grad_ptr<R>(c, 5), c->P,
-            c->st);
+            c->st, c->minslots);
     });
+}
+
+// Fold the per-slot constraint minima into each problem's DevState.
+int enqueue_minima(lulesh_ctx *c) {
+    return launch(c, T_MINIMA, [&] {
+        k_reduce_minima<<<c->nprob, kMinSlots, 0, c->stream>>>(c->st, c->minslots);
+    });
+}
+
+template <typename R>
+int enqueue_material_and_minima(lulesh_ctx *c) {
+    const int rc = enqueue_material<R>(c);
+    return rc ? rc : enqueue_minima(c);
 }
 
 int enqueue_cycle_body(lulesh_ctx *c) {
@@ -977,11 +1041,11 @@ int enqueue_cycle_body(lulesh_ctx *c) {
     if (c->precision == LULESH_PRECISION_MIXED) {
         if ((rc = enqueue_nodal<float>(c))) return rc;
         if ((rc = enqueue_elements<float>(c))) return rc;
-        return enqueue_material<float>(c);
+        return enqueue_material_and_minima<float>(c);
     }
     if ((rc = enqueue_nodal<double>(c))) return rc;
     if ((rc = enqueue_elements<double>(c))) return rc;
-    return enqueue_material<double>(c);
+    return enqueue_material_and_minima<double>(c);
 }
 
 int enqueue_cycle(lulesh_ctx *c) {
@@ -1199,7 +1263,8 @@ int create_impl(const lulesh_domain_view *const *views, int nprob, int precision
     c->totN = totN;
     c->P = params_of(views[0]);
     c->hgcoef = views[0]->hgcoef;
-    c->timers = {{"time_increment"}, {"nodal"}, {"nodes"}, {"elements"}, {"material"}};
+    c->timers = {{"time_increment"}, {"nodal"}, {"nodes"}, {"elements"}, {"material"},
+                 {"minima"}};
     int rc;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
@@ -1303,6 +1368,12 @@ int create_impl(const lulesh_domain_view *const *views, int nprob, int precision
             }
         }
         if (fits && ((rc = dalloc(c, &c->erep, E)) || (rc = upload(c, c->erep, rep.data(), E))))
+            return fail(rc);
+    }
+    {
+        const size_t ns = size_t(nprob) * kMinSlots * kSlotU64;
+        std::vector<unsigned long long> init(ns, kMinInit);
+        if ((rc = dalloc(c, &c->minslots, ns)) || (rc = upload(c, c->minslots, init.data(), ns)))
             return fail(rc);
     }
     if (!structured) {
@@ -1478,13 +1549,15 @@ int lulesh_hip_phase(lulesh_ctx *c, int phase, double deltatime) {
         rc = mixed ? enqueue_elements<float>(c) : enqueue_elements<double>(c);
         break;
     case LULESH_PHASE_MATERIAL:
-        rc = mixed ? enqueue_material<float>(c) : enqueue_material<double>(c);
+        rc = mixed ? enqueue_material_and_minima<float>(c)
+                   : enqueue_material_and_minima<double>(c);
         break;
     default:
         k_constraints<<<grid_for(c->totE, kBlock), kBlock, 0, c->stream>>>(
             int32_t(c->totE), c->f[LULESH_F_SS], c->f[LULESH_F_VDOV], c->f[LULESH_F_AREALG], c->P,
-            c->st);
+            c->minslots);
         HIP_TRY(c, hipGetLastError());
+        rc = enqueue_minima(c);
     }
     if (rc) return rc;
     if ((rc = read_states(c))) return rc;

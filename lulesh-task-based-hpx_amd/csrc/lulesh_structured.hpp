@@ -113,10 +113,9 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
     double px = 0.0, py = 0.0, pz = 0.0; // partial sums of the node in the current plane
 
     const int own = (jj + 1) * (kPatch + 1) + (ii + 1); // node (J, I) in the patch node array
-    auto finish_node = [&](int K) {
+    auto finish_node = [&](int K, double mass) {
         const int g = (K * n + J) * n + I;
         const int slot = K & 1;
-        const double mass = nodalMass[g];
         double ax = px / mass, ay = py / mass, az = pz / mass;
         if (STORE_F) {
             fo.fx[g] = px;
@@ -221,6 +220,12 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
     store_plane(kbeg);
     fetch_plane(kbeg + 1, kbeg < kend);
     fetch_elem(kbeg, kbeg < kend);
+    // nodal mass of this thread's node in the plane finished by the current
+    // step (fetched one step ahead, like the node planes)
+    auto fetch_mass = [&](int K) {
+        return ownsNode && K <= s ? nodalMass[(K * n + J) * n + I] : 0.0;
+    };
+    double mcur = fetch_mass(kbeg);
     for (int k = kbeg; k < kend; ++k) {
         // plane k is in its slot; plane k+1 (fetched during the previous
         // step) goes over plane k-1, whose last readers (finish_node) passed
@@ -318,9 +323,10 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
             for (int c = 0; c < 24; ++c) up[c] = 0.0;
         }
         __builtin_amdgcn_sched_barrier(0); // keep the prefetch out of the element work
-        // the next step's plane and element scalars
+        // the next step's plane, element scalars and nodal mass
         fetch_plane(k + 2, k + 1 < kend);
         fetch_elem(k + 1, k + 1 < kend);
+        const double mnext = fetch_mass(k + 1);
         __syncthreads();
         // ---- node plane k ends with corners 2,3,1,0 ----------------------------
         if (ownsNode && k >= K0) {
@@ -328,7 +334,7 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
             if (v01) add(3, tid + 1);
             if (v10) add(1, tid + kPatch);
             if (v11) add(0, tid + kPatch + 1);
-            finish_node(k);
+            finish_node(k, mcur);
         }
         __syncthreads();
         if (elemCol) {
@@ -345,9 +351,10 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
             if (v10) add(1, tid + kPatch);    // corner 5 of element (J, I-1)
             if (v11) add(0, tid + kPatch + 1); // corner 4 of element (J, I)
         }
+        mcur = mnext;
     }
     // The top node plane (K = s) has no element plane above it.
-    if (ownsNode && K1 == n && kend == s && s >= K0) finish_node(s);
+    if (ownsNode && K1 == n && kend == s && s >= K0) finish_node(s, mcur);
 }
 
 } // namespace lulesh_dev
