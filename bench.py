@@ -64,19 +64,21 @@ def kernel_bytes(name, E, N, mixed=False, material_mode=0):
 
 
 TRAFFIC_FILE = "profiles/hbm_traffic_sedov_s150_fp64.md"
+PROF_CYCLES = 64  # cycles of the per-kernel event timing pass
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` at s=150 FP64, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); None if the
-    summary is absent."""
+def measured(key, kernel):
+    """A per-launch figure of `kernel` at s=150 FP64 from the committed
+    rocprofv3 summary: key "traffic" = HBM bytes (FETCH_SIZE + WRITE_SIZE,
+    copy-calibrated), key "valu_busy" = fraction of time the VALUs issue.
+    None if the summary is absent."""
     path = os.path.join(ROOT, TRAFFIC_FILE)
     if not os.path.exists(path):
         return None
     for line in open(path):
         f = line.split()
-        if len(f) == 3 and f[0] == "traffic" and f[1] == kernel:
-            return int(f[2])
+        if len(f) == 3 and f[0] == key and f[1] == kernel:
+            return float(f[2])
     return None
 
 
@@ -171,7 +173,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--batch-its", type=int, default=1000)
-    ap.add_argument("--tune-cycles", type=int, default=6)
+    ap.add_argument("--tune-cycles", type=int, default=0,
+                    help="autotune the s=150 legs first (0: the library's default launch shapes)")
+    ap.add_argument("--batch-tune-cycles", type=int, default=6,
+                    help="autotune the batch leg first (0: default launch shapes)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -197,22 +202,21 @@ def main():
     zcps = E * cycles / elapsed
     print(f"[bench] fp64: {cycles} cycles in {elapsed:.3f}s -> {zcps:.4e} zc/s",
           file=sys.stderr, flush=True)
-    dev.download_view()
-    e0_fp64 = float(dom.field("e")[0])
-    rep64 = dom.report()
 
-    # Per-kernel HIP-event timings: the next `prof_cycles` cycles run with an
+    # Per-kernel HIP-event timings: straight after the timed region (the GPU
+    # still in its steady state), the next PROF_CYCLES cycles run with an
     # event pair around every kernel launch on the context's stream (the
-    # kernels and tuning of the timed region, launched one by one).
+    # kernels and tuning of the timed region, launched one by one, back to
+    # back; the events are read when the stream next synchronises).
     dev.set_timing(True)
-    ts2 = dom.time_state()
-    ts2.time, ts2.deltatime, ts2.dtcourant, ts2.dthydro, ts2.cycle = (
-        ts.time, ts.deltatime, ts.dtcourant, ts.dthydro, ts.cycle)
-    prof_cycles = 32
-    rc = dev.run(ts.cycle + prof_cycles, ts2)
+    rc = dev.run(ts.cycle + PROF_CYCLES, ts)
     assert rc == 0, rc
     kt = dev.kernel_times()
     dev.set_timing(False)
+    prof_cycles = PROF_CYCLES
+    dev.download_view()
+    e0_fp64 = float(dom.field("e")[0])
+    rep64 = dom.report()
     mode = dev.tuning().material_mode
     kernels = {}
     for name, (avg_ms, n) in kt.items():
@@ -226,7 +230,9 @@ def main():
     dom_k = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches_per_cycle"])
     dk = kernels[dom_k]
     achieved = dk["bytes_per_launch"] / (dk["avg_ms"] * 1e-3) / 1e9
-    traffic = measured_traffic(dom_k) if (s, nr) == (150, 11) else None
+    prof = (s, nr) == (150, 11)
+    traffic = measured("traffic", dom_k) if prof else None
+    valu_busy = measured("valu_busy", dom_k) if prof else None
     ms_cycle_events = sum(k["avg_ms"] * k["launches_per_cycle"] for k in kernels.values())
     dev.close()
 
@@ -236,6 +242,8 @@ def main():
         domm = pyl.Domain(s, nr)
         devm = pyl.Device(domm, pyl.MIXED)
         el_m, tsm, _ = timed_run(pyl, devm, domm, args.warmup, args.steps, args.tune_cycles)
+        rc = devm.run(tsm.cycle + PROF_CYCLES, tsm)  # to the cycle the FP64 state is read at
+        assert rc == 0, rc
         devm.download_view()
         e0m = float(domm.field("e")[0])
         mixed = {"value": E * args.steps / el_m, "ms_per_step": 1e3 * el_m / args.steps,
@@ -248,7 +256,7 @@ def main():
     batch = None
     if not args.no_batch:
         print(f"[bench] batch of {args.batch}", file=sys.stderr, flush=True)
-        batch = batch_bench(pyl, args.batch, args.batch_its, 0, args.tune_cycles, pyl.FP64)
+        batch = batch_bench(pyl, args.batch, args.batch_its, 0, args.batch_tune_cycles, pyl.FP64)
         print(f"[bench] batch: {batch['value']:.4e} zc/s", file=sys.stderr, flush=True)
 
     # ---- CPU baseline ----------------------------------------------------
@@ -280,8 +288,10 @@ def main():
                    "s": s, "regions": nr, "elements": E},
         "roofline": {"bound": "hbm", "kernel": dom_k, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "traffic_source": TRAFFIC_FILE if traffic else None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": int(traffic) if traffic else None,
+                     "valu_busy": valu_busy,
+                     "source_of_traffic_and_valu_busy": TRAFFIC_FILE if traffic else None},
         "model_bytes_per_zone_cycle": round(bsz, 1),
         "model_hbm_frac": round(bsz * zcps / (HBM_PEAK_GBS * 1e9), 4),
         "kernels": kernels,

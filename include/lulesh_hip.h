@@ -170,8 +170,10 @@ double lulesh_hip_last_run_ms(const lulesh_ctx *ctx);
  * task size (--task-size / --elems-per-task, lulesh.cc:2267-2335). */
 typedef struct lulesh_tuning {
     int32_t zplanes;          /* node planes per z-march workgroup (0 = sized to fill the GPU) */
-    int32_t block_elem;       /* threads per block of the kinematics kernel: 64, 128, 256 */
-    int32_t block_material;   /* threads per block of the material/EOS kernel: 64, 128, 256 */
+    int32_t block_elem;       /* threads per block of the kinematics kernel: 64, 128, 256
+                                 (other values: 128) */
+    int32_t block_material;   /* threads per block of the material/EOS kernel: 64, 128, 256
+                                 (other values: 128) */
     int32_t cycles_per_graph; /* cycles captured per HIP graph (0 = by problem size) */
     int32_t material_mode;    /* 0: element order, EOS repetitions balanced over each wave;
                                  1: region lists grouped by repetition count */
@@ -181,7 +183,8 @@ int lulesh_hip_set_tuning(lulesh_ctx *ctx, const lulesh_tuning *tuning);
 int lulesh_hip_get_tuning(const lulesh_ctx *ctx, lulesh_tuning *tuning);
 /* Autotuner: times `cycles` cycles for each candidate launch configuration
  * (coordinate descent over zplanes, block_elem, block_material,
- * cycles_per_graph) starting from a snapshot of the current state, restores
+ * cycles_per_graph, material_mode; each candidate is run twice from the
+ * snapshot, the second run timed) starting from a snapshot of the current state, restores
  * the state, and keeps the fastest.  Returns it and its ms per cycle. */
 int lulesh_hip_autotune(lulesh_ctx *ctx, int32_t cycles, lulesh_tuning *best,
                         double *best_ms_per_cycle);
