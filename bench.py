@@ -196,7 +196,9 @@ def main():
     # ---- FP64 headline -------------------------------------------------
     print(f"[bench] building Domain s={s}", file=sys.stderr, flush=True)
     dom = pyl.Domain(s, nr)
-    dev = pyl.Device(dom, pyl.FP64)
+    t_up = time.perf_counter()
+    dev = pyl.Device(dom, pyl.FP64)  # allocation, upload of every field, layout checks
+    t_up = time.perf_counter() - t_up
     elapsed, ts, tuning = timed_run(pyl, dev, dom, args.warmup, args.steps, args.tune_cycles)
     cycles = args.steps
     zcps = E * cycles / elapsed
@@ -214,7 +216,9 @@ def main():
     kt = dev.kernel_times()
     dev.set_timing(False)
     prof_cycles = PROF_CYCLES
+    t_down = time.perf_counter()
     dev.download_view()
+    t_down = time.perf_counter() - t_down
     e0_fp64 = float(dom.field("e")[0])
     rep64 = dom.report()
     mode = dev.tuning().material_mode
@@ -296,6 +300,8 @@ def main():
         "model_hbm_frac": round(bsz * zcps / (HBM_PEAK_GBS * 1e9), 4),
         "kernels": kernels,
         "ms_per_cycle_kernel_sum": round(ms_cycle_events, 4),
+        "value_incl_host_transfers": E * cycles / (elapsed + t_up + t_down),
+        "host_transfer_s": {"create_and_upload": round(t_up, 4), "download": round(t_down, 4)},
         "origin_energy": e0_fp64,
         "symmetry_max_abs_diff": float(rep64[1]),
         "tuning": tuning,
