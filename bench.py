@@ -250,8 +250,10 @@ def main():
         assert rc == 0, rc
         devm.download_view()
         e0m = float(domm.field("e")[0])
+        repm = domm.report()
         mixed = {"value": E * args.steps / el_m, "ms_per_step": 1e3 * el_m / args.steps,
                  "origin_energy_rel_err_vs_fp64": abs(e0m - e0_fp64) / abs(e0_fp64),
+                 "symmetry_max_abs_diff": float(repm[1]), "symmetry_max_rel_diff": float(repm[3]),
                  "cycles": tsm.cycle}
         devm.close()
         domm.close()
@@ -304,6 +306,7 @@ def main():
         "host_transfer_s": {"create_and_upload": round(t_up, 4), "download": round(t_down, 4)},
         "origin_energy": e0_fp64,
         "symmetry_max_abs_diff": float(rep64[1]),
+        "symmetry_max_rel_diff": float(rep64[3]),
         "tuning": tuning,
         "mixed": mixed,
         "batch": batch,

@@ -1,0 +1,81 @@
+"""Runs at the benchmark sizes: s=150 with 11 regions (the headline
+configuration) and s=250 with 21 regions and -c 1 (region imbalance at
+scale).
+
+* s=150: 300 cycles on the GPU against the oracle (OpenMP) on the same
+  inputs: origin energy within 1e-9 relative and symmetry diffs
+  (MaxAbsDiff, MaxRelDiff of the x/y/z permutation check) no worse than the
+  oracle's; the first two cycles bitwise (no cbrt-dependent term reaches the
+  state before cycle 3).
+* s=250: the first two cycles bitwise against the oracle, then 500 cycles of
+  FP64 and of the FP32-mixed form: mixed origin energy within 1e-6 of FP64,
+  and the FP64 symmetry diffs at rounding level.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle_lib import OracleDomain, set_threads
+
+pytestmark = pytest.mark.gpu
+
+STATE = ["x", "y", "z", "xd", "yd", "zd", "e", "p", "q", "v", "ss", "delv", "vdov", "arealg",
+         "ql", "qq", "vnew"]
+
+
+def banner(msg):
+    print(f"\n[fullsize] {msg}", flush=True)
+
+
+def threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+@pytest.mark.parametrize("s,nr", [(150, 11), (250, 21)])
+def test_first_two_cycles_bitwise(pyl, s, nr):
+    banner(f"s={s} nr={nr}: two cycles vs oracle")
+    set_threads(threads())
+    ora = OracleDomain(s, nr)
+    assert ora.run(2) == 0
+    dom, dev, ts = pyl.run_device(s, 2, nr)
+    assert ts.cycle == ora.cycle == 2
+    for f in STATE:
+        np.testing.assert_array_equal(dom.field(f), ora.f64(f), err_msg=f)
+    dev.close()
+    ora.close()
+
+
+def test_s150_300_cycles_vs_oracle(pyl):
+    banner("s=150: 300 cycles vs oracle")
+    set_threads(threads())
+    ora = OracleDomain(150, 11)
+    assert ora.run(300) == 0
+    dom, dev, ts = pyl.run_device(150, 300, 11)
+    assert ts.cycle == ora.cycle == 300
+    e0, oe0 = dom.field("e")[0], ora.f64("e")[0]
+    assert abs(e0 - oe0) <= 1e-9 * abs(oe0)
+    rep, orep = dom.report(), ora.report()
+    print(f"[fullsize] gpu report {rep.tolist()} oracle {orep.tolist()}", flush=True)
+    assert rep[1] <= max(2 * orep[1], 1e-12 * abs(e0))  # MaxAbsDiff
+    assert rep[3] <= max(2 * orep[3], 1e-14)            # MaxRelDiff
+    dev.close()
+    ora.close()
+
+
+def test_s250_fp64_and_mixed(pyl):
+    banner("s=250, 21 regions: 500 cycles FP64 and mixed")
+    d64, dev64, t64 = pyl.run_device(250, 500, 21)
+    rep64 = d64.report()
+    e64 = d64.field("e")[0]
+    dev64.close()
+    dm, devm, tm = pyl.run_device(250, 500, 21, precision=pyl.MIXED)
+    repm = dm.report()
+    em = dm.field("e")[0]
+    print(f"[fullsize] fp64 e0 {e64!r} report {rep64.tolist()}; mixed e0 {em!r} report "
+          f"{repm.tolist()} rel {abs(em - e64) / abs(e64):.3e}", flush=True)
+    assert t64.cycle == tm.cycle == 500
+    assert abs(em - e64) <= 1e-6 * abs(e64)
+    assert rep64[3] <= 1e-10   # FP64 symmetry: rounding order only
+    assert repm[3] <= 1e-4     # FP32 hourglass / Q gradients
+    devm.close()
