@@ -67,6 +67,30 @@ __device__ __forceinline__ double cbrt_(double a) { return ::cbrt(a); }
 __device__ __forceinline__ float cbrt_(float a) { return ::cbrtf(a); }
 __device__ __forceinline__ double fabs_(double a) { return __builtin_fabs(a); }
 
+// Square root of a finite x >= 2^-767: the reciprocal square root
+// estimate refined by two steps of the coupled Newton iteration for
+// (sqrt x, 1/(2 sqrt x)) (Markstein, "IA-64 and Elementary Functions", 2000).
+// The result is correctly rounded; it is the same operation sequence that
+// __builtin_sqrt expands to, without the denormal-range scaling and the
+// zero/infinity selects, which do not apply in this range, so the two agree
+// bit for bit on it.
+__device__ __forceinline__ double sqrt_normal(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+
+// Square root of a face-area norm plus ptiny (>= 1e-36): sqrt_normal in FP64.
+__device__ __forceinline__ double sqrt_area(double x) { return sqrt_normal(x); }
+__device__ __forceinline__ float sqrt_area(float x) { return sqrt_(x); }
+
 // Keep the compiler from treating one repetition of an idempotent loop body
 // as equal to the previous one: the value is declared modified by an empty
 // asm statement (no instruction is emitted).
@@ -353,7 +377,7 @@ __device__ __forceinline__ void q_gradients(const T x[8], const T y[8], const T 
     ax = dyi * dzj - dzi * dyj;
     ay = dzi * dxj - dxi * dzj;
     az = dxi * dyj - dyi * dxj;
-    out[2] = vol / sqrt_(ax * ax + ay * ay + az * az + ptiny);
+    out[2] = vol / sqrt_area(ax * ax + ay * ay + az * az + ptiny);
     ax *= norm; ay *= norm; az *= norm;
     dxv = T(0.25) * ((xv[4] + xv[5] + xv[6] + xv[7]) - (xv[0] + xv[1] + xv[2] + xv[3]));
     dyv = T(0.25) * ((yv[4] + yv[5] + yv[6] + yv[7]) - (yv[0] + yv[1] + yv[2] + yv[3]));
@@ -363,7 +387,7 @@ __device__ __forceinline__ void q_gradients(const T x[8], const T y[8], const T 
     ax = dyj * dzk - dzj * dyk;
     ay = dzj * dxk - dxj * dzk;
     az = dxj * dyk - dyj * dxk;
-    out[0] = vol / sqrt_(ax * ax + ay * ay + az * az + ptiny);
+    out[0] = vol / sqrt_area(ax * ax + ay * ay + az * az + ptiny);
     ax *= norm; ay *= norm; az *= norm;
     dxv = T(0.25) * ((xv[1] + xv[2] + xv[6] + xv[5]) - (xv[0] + xv[3] + xv[7] + xv[4]));
     dyv = T(0.25) * ((yv[1] + yv[2] + yv[6] + yv[5]) - (yv[0] + yv[3] + yv[7] + yv[4]));
@@ -373,7 +397,7 @@ __device__ __forceinline__ void q_gradients(const T x[8], const T y[8], const T 
     ax = dyk * dzi - dzk * dyi;
     ay = dzk * dxi - dxk * dzi;
     az = dxk * dyi - dyk * dxi;
-    out[1] = vol / sqrt_(ax * ax + ay * ay + az * az + ptiny);
+    out[1] = vol / sqrt_area(ax * ax + ay * ay + az * az + ptiny);
     ax *= norm; ay *= norm; az *= norm;
     dxv = T(-0.25) * ((xv[0] + xv[1] + xv[5] + xv[4]) - (xv[3] + xv[2] + xv[6] + xv[7]));
     dyv = T(-0.25) * ((yv[0] + yv[1] + yv[5] + yv[4]) - (yv[3] + yv[2] + yv[6] + yv[7]));
@@ -438,26 +462,6 @@ __device__ __forceinline__ double pressure(double &bvc, double &pbvc, double e_o
     if (fabs_(newval) < P.p_cut || vnewc >= P.eosvmax) newval = 0.0;
     if (newval < P.pmin) newval = P.pmin;
     return newval;
-}
-
-// Square root of a finite x >= 2^-767: the reciprocal square root
-// estimate refined by two steps of the coupled Newton iteration for
-// (sqrt x, 1/(2 sqrt x)) (Markstein, "IA-64 and Elementary Functions", 2000).
-// The result is correctly rounded; it is the same operation sequence that
-// __builtin_sqrt expands to, without the denormal-range scaling and the
-// zero/infinity selects, which do not apply in this range, so the two agree
-// bit for bit on it.
-__device__ __forceinline__ double sqrt_normal(double x) {
-    const double y = __builtin_amdgcn_rsq(x);
-    double g = x * y;
-    double h = y * 0.5;
-    const double r = __builtin_fma(-h, g, 0.5);
-    g = __builtin_fma(g, r, g);
-    h = __builtin_fma(h, r, h);
-    double d = __builtin_fma(-g, g, x);
-    g = __builtin_fma(d, h, g);
-    d = __builtin_fma(-g, g, x);
-    return __builtin_fma(d, h, g);
 }
 
 // sqrt with the reference's floor, lulesh.cc:1779-1783; the square root is

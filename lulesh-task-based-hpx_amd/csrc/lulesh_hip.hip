@@ -289,7 +289,8 @@ __global__ __launch_bounds__(kBlock) void k_elems(
     if (STRUCT) {
         // element (k,j,i) -> its eight nodes, lulesh-init.cc:236-244
         const int s = pd.s, nn = s + 1;
-        const int k = li / (s * s), r = li - k * s * s, j = r / s, ci = r - j * s;
+        const int k = int((double(li) + 0.5) * pd.inv_ss), r = li - k * s * s;
+        const int j = int((double(r) + 0.5) * pd.inv_s), ci = r - j * s;
         const int b = pd.nodeOff + (k * nn + j) * nn + ci;
         n[0] = b; n[1] = b + 1; n[2] = b + nn + 1; n[3] = b + nn;
         n[4] = b + nn * nn; n[5] = b + nn * nn + 1; n[6] = b + nn * nn + nn + 1;
@@ -906,6 +907,8 @@ int configure(lulesh_ctx *c) {
         d.nodeOff = ph.nodeOff;
         const int n = ph.s + 1;
         d.tiles = (n + kTile - 1) / kTile;
+        d.inv_s = 1.0 / double(ph.s);
+        d.inv_ss = 1.0 / (double(ph.s) * double(ph.s));
         d.zplanes = std::min(t.zplanes, n);
         d.zBlock0 = int32_t(zb.size());
         const int64_t nz = int64_t(d.tiles) * d.tiles * ((n + d.zplanes - 1) / d.zplanes);

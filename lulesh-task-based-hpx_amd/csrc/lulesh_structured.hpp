@@ -50,6 +50,11 @@ struct ProbDesc {
     int32_t zplanes, tiles;   // z-march: node planes per workgroup, node tiles per edge
     int32_t zBlock0, eBlock0; // first workgroup of this problem in the z-march / element grids
     int32_t mBlock0, pad;     // first workgroup in the element-order material grid
+    // 1/s and 1/s^2 rounded to double: element (k, j, i) from its index li
+    // as floor((li + 0.5) * (1/d)), exact because the quotient stays at
+    // least 0.5/s^2 (> 2^-20 for s <= 645) away from an integer while the
+    // two roundings move it by less than 2^-40
+    double inv_s, inv_ss;
 };
 
 // Two waves per SIMD: the FP64 form needs about 260 VGPRs with the register
