@@ -452,6 +452,16 @@ __device__ __forceinline__ void q_terms(double phixi, double phieta, double phiz
              dzeta * dzeta * (1. - phizeta * phizeta));
 }
 
+// Lower clamp `if (v < lo) v = lo` of the EOS (lulesh.cc:1685-1687, 1758-1760,
+// 1799-1801, 1839-1841) as one v_max_f64.  Equal to the compare-and-assign for
+// every non-NaN v except v = -0.0 against lo = +0.0 (max gives +0.0).  That
+// pair does not occur with the reference constants: emin = -1e15, and a
+// pressure reaches the pmin = 0 clamp only after |p| < p_cut (1e-7) has
+// replaced every zero by +0.0.
+__device__ __forceinline__ double clamp_below(double v, double lo) {
+    return __builtin_fmax(v, lo);
+}
+
 // CalcPressureForElemsTask for one element, lulesh.cc:1671-1690
 __device__ __forceinline__ double pressure(double &bvc, double &pbvc, double e_old,
                                            double compression, double vnewc, const Params &P) {
@@ -460,7 +470,7 @@ __device__ __forceinline__ double pressure(double &bvc, double &pbvc, double e_o
     pbvc = cls;
     double newval = bvc * e_old;
     if (fabs_(newval) < P.p_cut || vnewc >= P.eosvmax) newval = 0.0;
-    if (newval < P.pmin) newval = P.pmin;
+    newval = clamp_below(newval, P.pmin);
     return newval;
 }
 
@@ -498,7 +508,7 @@ __device__ __forceinline__ void eos(double e_old, double delvc, double p_old, do
     }
     double bvc, pbvc;
     double e_new = e_old - 0.5 * delvc * (p_old + q_old) + 0.5 * work;
-    if (e_new < P.emin) e_new = P.emin;
+    e_new = clamp_below(e_new, P.emin);
     const double pHalfStep = pressure(bvc, pbvc, e_new, compHalfStep, vnewc, P);
     double q_new;
     {
@@ -515,7 +525,7 @@ __device__ __forceinline__ void eos(double e_old, double delvc, double p_old, do
     {
         double nv = e_new + 0.5 * work;
         if (fabs_(nv) < P.e_cut) nv = 0.0;
-        if (nv < P.emin) nv = P.emin;
+        nv = clamp_below(nv, P.emin);
         e_new = nv;
     }
     double p_new = pressure(bvc, pbvc, e_new, compression, vnewc, P);
@@ -532,7 +542,7 @@ __device__ __forceinline__ void eos(double e_old, double delvc, double p_old, do
         e_new = e_new - (7.0 * (p_old + q_old) - 8.0 * (pHalfStep + q_new) + (p_new + q_tilde)) *
                             delvc * sixth;
         if (fabs_(e_new) < P.e_cut) e_new = 0.;
-        if (e_new < P.emin) e_new = P.emin;
+        e_new = clamp_below(e_new, P.emin);
     }
     p_new = pressure(bvc, pbvc, e_new, compression, vnewc, P);
     // The last q of CalcEnergyForElems (lulesh.cc:1848-1866) and the sound
