@@ -33,7 +33,8 @@ struct DevState {
     double time, deltatime, stoptime, dtfixed, deltatimemultlb, deltatimemultub, dtmax;
     unsigned long long dtc_bits, dth_bits; // constraint minima (bit patterns of positive doubles)
     int32_t cycle, its, active, error;
-    int32_t cur, pad;
+    int32_t cur;
+    int32_t pending; // the last cycle's constraint minima are still in the slots
 };
 
 // Both halves of the node double buffer: b[h][0..5] = x, y, z, xd, yd, zd.

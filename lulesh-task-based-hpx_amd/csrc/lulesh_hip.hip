@@ -60,18 +60,87 @@ inline unsigned long long host_dbits(double v) {
     return b;
 }
 
+// The Courant / hydro minima of a cycle (lulesh.cc:1938-1974, 2204-2239) are
+// reduced in two levels.  Each workgroup takes the minimum of its elements
+// and merges it with an atomic min into one of kMinSlots slots of its problem
+// (one 128-byte line each, so the atomics of different workgroups go to
+// different addresses instead of queueing on one), and k_reduce_minima then
+// folds the slots into the problem's DevState and re-arms them.  Minima are
+// exact, so the grouping does not change the result.  The values are
+// positive doubles, whose bit patterns order like the values.
+constexpr int kMinSlots = 256;
+constexpr int kSlotU64 = 16;
+constexpr unsigned long long kMinInit = 0x4415AF1D78B58C40ull; // 1.0e+20
+
+// Minimum over the kMinSlots slots of one problem, by a workgroup of
+// kMinSlots threads; the slots are re-armed to 1e20.  The result is valid in
+// thread 0.
+__device__ __forceinline__ void fold_slots(unsigned long long *prob_slots,
+                                           unsigned long long &c, unsigned long long &h) {
+    unsigned long long *sl = prob_slots + size_t(threadIdx.x) * kSlotU64;
+    c = sl[0];
+    h = sl[1];
+    sl[0] = kMinInit;
+    sl[1] = kMinInit;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long oc = __shfl_xor(c, off, 64);
+        const unsigned long long oh = __shfl_xor(h, off, 64);
+        c = oc < c ? oc : c;
+        h = oh < h ? oh : h;
+    }
+    __shared__ unsigned long long rc[kMinSlots / 64], rh[kMinSlots / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        rc[wave] = c;
+        rh[wave] = h;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < kMinSlots / 64; ++w) {
+            c = rc[w] < c ? rc[w] : c;
+            h = rh[w] < h ? rh[w] : h;
+        }
+}
+
+// One workgroup per problem: if a cycle's minima are pending, dtcourant /
+// dthydro = minimum over the problem's slots (used after the last cycle of a
+// run and after host-driven cycles and phases).
+__global__ __launch_bounds__(kMinSlots) void k_reduce_minima(DevState *states,
+                                                             unsigned long long *slots) {
+    DevState *st = states + blockIdx.x;
+    if (!st->pending) return; // uniform across the workgroup
+    unsigned long long c, h;
+    fold_slots(slots + size_t(blockIdx.x) * kMinSlots * kSlotU64, c, h);
+    if (threadIdx.x == 0) {
+        st->dtc_bits = c;
+        st->dth_bits = h;
+        st->pending = 0;
+    }
+}
+
 // ---------------------------------------------------------------------------
-// TimeIncrement on the device, lulesh.cc:182-229, one thread per problem.
-// The loop condition of lulesh.cc:2374-2375 decides whether the cycle runs.
+// TimeIncrement on the device, lulesh.cc:182-229, one workgroup per problem:
+// the workgroup first folds the previous cycle's constraint minima out of
+// the slots, then thread 0 steps the time.  The loop condition of
+// lulesh.cc:2374-2375 decides whether the cycle runs.
 // ---------------------------------------------------------------------------
-__global__ void k_time_increment(DevState *states, int32_t nprob) {
-    const int pi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pi >= nprob) return;
-    DevState *st = states + pi;
+__global__ __launch_bounds__(kMinSlots) void k_time_increment(DevState *states,
+                                                              unsigned long long *slots) {
+    DevState *st = states + blockIdx.x;
+    const bool fold = st->pending != 0; // uniform across the workgroup
+    unsigned long long c = 0, h = 0;
+    if (fold) fold_slots(slots + size_t(blockIdx.x) * kMinSlots * kSlotU64, c, h);
+    if (threadIdx.x != 0) return;
     DevState s = *st;
+    if (fold) {
+        s.dtc_bits = st->dtc_bits = c;
+        s.dth_bits = st->dth_bits = h;
+    }
     if (s.active) st->cur = s.cur ^ 1; // adopt the node state the last cycle wrote
     const bool run = !s.error && (s.time < s.stoptime) && (s.cycle < s.its);
     st->active = run ? 1 : 0;
+    st->pending = run ? 1 : 0;
     if (!run) return;
     double targetdt = s.stoptime - s.time;
     double deltatime = s.deltatime;
@@ -110,6 +179,7 @@ __global__ void k_begin_cycle(DevState *st, double deltatime, int reset_minima, 
     if (adopt && st->active) st->cur ^= 1;
     st->deltatime = deltatime;
     st->active = 1;
+    st->pending = reset_minima ? 1 : 0; // phases that produce minima
     st->error = 0;
     if (reset_minima) {
         st->dtc_bits = dbits(1.0e+20);
@@ -368,18 +438,6 @@ struct MatSeg {
     int32_t prob, rep, offset, count, block0;
 };
 
-// The Courant / hydro minima of a cycle (lulesh.cc:1938-1974, 2204-2239) are
-// reduced in two levels.  Each workgroup takes the minimum of its elements
-// and merges it with an atomic min into one of kMinSlots slots of its problem
-// (one 128-byte line each, so the atomics of different workgroups go to
-// different addresses instead of queueing on one), and k_reduce_minima then
-// folds the slots into the problem's DevState and re-arms them.  Minima are
-// exact, so the grouping does not change the result.  The values are
-// positive doubles, whose bit patterns order like the values.
-constexpr int kMinSlots = 256;
-constexpr int kSlotU64 = 16;
-constexpr unsigned long long kMinInit = 0x4415AF1D78B58C40ull; // 1.0e+20
-
 __device__ __forceinline__ void block_min_atomic(double c, double h,
                                                  unsigned long long *prob_slots) {
     __shared__ double sc[kBlock / 64], sh[kBlock / 64];
@@ -407,39 +465,6 @@ __device__ __forceinline__ void block_min_atomic(double c, double h,
     }
 }
 
-// One workgroup of kMinSlots threads per problem: dtcourant / dthydro of the
-// cycle = minimum over the slots, which are reset to 1e20 for the next cycle.
-__global__ __launch_bounds__(kMinSlots) void k_reduce_minima(DevState *states,
-                                                             unsigned long long *slots) {
-    DevState *st = states + blockIdx.x;
-    if (!st->active) return;
-    unsigned long long *sl = slots + (size_t(blockIdx.x) * kMinSlots + threadIdx.x) * kSlotU64;
-    unsigned long long c = sl[0], h = sl[1];
-    sl[0] = kMinInit;
-    sl[1] = kMinInit;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long oc = __shfl_xor(c, off, 64);
-        const unsigned long long oh = __shfl_xor(h, off, 64);
-        c = oc < c ? oc : c;
-        h = oh < h ? oh : h;
-    }
-    __shared__ unsigned long long rc[kMinSlots / 64], rh[kMinSlots / 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) {
-        rc[wave] = c;
-        rh[wave] = h;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < kMinSlots / 64; ++w) {
-            c = rc[w] < c ? rc[w] : c;
-            h = rh[w] < h ? rh[w] : h;
-        }
-        st->dtc_bits = c;
-        st->dth_bits = h;
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Material: CalcMonotonicQRegionForElemsAndApplyInitTask (lulesh.cc:1420-1669),
@@ -1067,21 +1092,23 @@ int enqueue_material_and_minima(lulesh_ctx *c) {
     return rc ? rc : enqueue_minima(c);
 }
 
+// The three phases of a cycle; the constraint minima stay in the slots
+// (pending) until the next k_time_increment or k_reduce_minima folds them.
 int enqueue_cycle_body(lulesh_ctx *c) {
     int rc;
     if (c->precision == LULESH_PRECISION_MIXED) {
         if ((rc = enqueue_nodal<float>(c))) return rc;
         if ((rc = enqueue_elements<float>(c))) return rc;
-        return enqueue_material_and_minima<float>(c);
+        return enqueue_material<float>(c);
     }
     if ((rc = enqueue_nodal<double>(c))) return rc;
     if ((rc = enqueue_elements<double>(c))) return rc;
-    return enqueue_material_and_minima<double>(c);
+    return enqueue_material<double>(c);
 }
 
 int enqueue_cycle(lulesh_ctx *c) {
     int rc = launch(c, T_TIME, [&] {
-        k_time_increment<<<grid_for(c->nprob, 64), 64, 0, c->stream>>>(c->st, c->nprob);
+        k_time_increment<<<c->nprob, kMinSlots, 0, c->stream>>>(c->st, c->minslots);
     });
     if (rc) return rc;
     return enqueue_cycle_body(c);
@@ -1147,11 +1174,14 @@ int run_loop(lulesh_ctx *c) {
         for (int pi = 0; pi < c->nprob; ++pi) any = any || still_running(c->hst[pi]);
         if (!any) break;
     }
+    // the last cycle's minima, for the time state handed back to the caller
+    if ((rc = enqueue_minima(c))) return rc;
     HIP_TRY(c, hipEventRecord(c->run1, c->stream));
     HIP_TRY(c, hipEventSynchronize(c->run1));
     float ms = 0;
     HIP_TRY(c, hipEventElapsedTime(&ms, c->run0, c->run1));
     c->last_run_ms = ms;
+    if ((rc = read_states(c))) return rc;
     for (int pi = 0; pi < c->nprob; ++pi)
         if (c->hst[pi].error) {
             c->err = "VolumeError in problem " + std::to_string(pi);
@@ -1551,7 +1581,7 @@ int lulesh_hip_leapfrog(lulesh_ctx *c, double deltatime, double *dtcourant, doub
         k_begin_cycle<<<1, 1, 0, c->stream>>>(c->st, deltatime, 1, 1);
     });
     if (rc) return rc;
-    if ((rc = enqueue_cycle_body(c))) return rc;
+    if ((rc = enqueue_cycle_body(c)) || (rc = enqueue_minima(c))) return rc;
     if ((rc = read_states(c))) return rc;
     if (c->hst->error) {
         c->err = "VolumeError";
@@ -1612,6 +1642,7 @@ int lulesh_hip_run(lulesh_ctx *c, int32_t max_cycles, lulesh_time_state *io) {
     s.deltatime = io->deltatime;
     s.dtc_bits = host_dbits(io->dtcourant);
     s.dth_bits = host_dbits(io->dthydro);
+    s.pending = 0; // the caller's minima replace any left in the slots
     s.cycle = io->cycle;
     s.its = max_cycles;
     s.error = 0;
