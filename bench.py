@@ -200,6 +200,7 @@ def main():
     dev = pyl.Device(dom, pyl.FP64)  # allocation, upload of every field, layout checks
     t_up = time.perf_counter() - t_up
     elapsed, ts, tuning = timed_run(pyl, dev, dom, args.warmup, args.steps, args.tune_cycles)
+    dev_ms = dev.last_run_ms()  # HIP events around the replayed graphs of the timed region
     cycles = args.steps
     zcps = E * cycles / elapsed
     print(f"[bench] fp64: {cycles} cycles in {elapsed:.3f}s -> {zcps:.4e} zc/s",
@@ -274,6 +275,11 @@ def main():
         cpu = {"value": v, "unit": "zone-cycles/s", "cores": threads, "kind": "port",
                "sample": f"OpenMP oracle, s={cs}, {nr} regions, {ncyc} cycles after cycle 1, "
                          f"{el:.1f}s wall, gcc -O3 -ffp-contract=off"}
+        # the same port on one core, a shorter sample
+        v1, ncyc1, el1 = cpu_baseline(cs, nr, max(2.0, args.cpu_seconds / 4), 1)
+        cpu["serial"] = {"value": v1, "cores": 1,
+                         "sample": f"serial oracle, s={cs}, {nr} regions, {ncyc1} cycles after "
+                                   f"cycle 1, {el1:.1f}s wall"}
 
     bsz = model_bytes_per_zone_cycle(s)
     out = {
@@ -284,6 +290,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": 1e3 * elapsed / cycles,
+        "device_ms_per_step": dev_ms / cycles,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
