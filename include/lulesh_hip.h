@@ -190,6 +190,26 @@ int lulesh_hip_autotune(lulesh_ctx *ctx, int32_t cycles, lulesh_tuning *best,
                         double *best_ms_per_cycle);
 
 /* ---------------------------------------------------------------------- */
+/* Set-up in HBM.  No counterpart in the reference, whose Domain constructor
+ * (lulesh-init.cc:10-183) builds the problem on the host.  Builds the Sedov
+ * problem of `s` elements per edge, `nr` regions, region balance and cost
+ * (the --s --r --b --c flags) directly on the device: every field equals the
+ * one lulesh_domain_create + lulesh_hip_create produce, bit for bit, without
+ * the O(E) host loops and the upload (only the region runs of the
+ * reference's rand() stream are drawn on the host).  Structured kernels only
+ * (LULESH_OPT_GENERIC is rejected) and element-order material.  The time
+ * state starts at the constructor's (lulesh_hip_get_time_state). */
+int lulesh_hip_create_sedov(int32_t s, int32_t nr, int32_t balance, int32_t cost,
+                            int precision_opts, lulesh_ctx **out);
+/* Time state of a single-problem context (time, dt, dtcourant, dthydro,
+ * cycle), as lulesh_hip_run would hand it back. */
+int lulesh_hip_get_time_state(lulesh_ctx *ctx, lulesh_time_state *ts);
+/* The report numbers of VerifyAndWriteFinalOutput (lulesh-util.cc:60-115)
+ * from the device fields: out[0] = e(0), out[1] = MaxAbsDiff,
+ * out[2] = TotalAbsDiff, out[3] = MaxRelDiff. */
+int lulesh_hip_report(lulesh_ctx *ctx, double out[4]);
+
+/* ---------------------------------------------------------------------- */
 /* Batched ensemble: many independent Sedov Domains in one context, advanced
  * together (one launch per kernel covers every problem) while each keeps its
  * own time step, time and cycle -- exactly as if run alone.  No counterpart

@@ -36,6 +36,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "domain.hpp"
 #include "lulesh_elem.hpp"
 #include "lulesh_hip.h"
 #include "lulesh_structured.hpp"
@@ -919,6 +920,7 @@ int configure(lulesh_ctx *c) {
     if (t.zplanes <= 0) t.zplanes = auto_zplanes(c);
     if (t.cycles_per_graph <= 0) t.cycles_per_graph = c->totE >= 1000000 ? 8 : 32;
     if (t.material_mode != 1 || !c->erep) t.material_mode = c->erep ? 0 : 1;
+    if (t.material_mode == 1 && !c->matList) t.material_mode = 0; // no region lists
     std::vector<int32_t> zb, eb, mb, ob;
     c->hpd.clear();
     c->hsegs.clear();
@@ -1561,6 +1563,91 @@ struct Snapshot {
 
 } // namespace
 
+// ---------------------------------------------------------------------------
+// Sedov set-up in HBM (the Domain constructor, lulesh-init.cc:10-183, single
+// rank, structured cube): the same values the host set-up produces, bit for
+// bit, without the O(E) host loops and the upload.
+// ---------------------------------------------------------------------------
+// Node coordinates from their lattice indices, lulesh-init.cc:207-227.
+__global__ void k_sedov_nodes(int32_t s, double *__restrict__ x, double *__restrict__ y,
+                              double *__restrict__ z) {
+    const int n = s + 1;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n * n * n) return;
+    const int col = g % n, row = (g / n) % n, plane = g / (n * n);
+    x[g] = 1.125 * double(col) / double(s);
+    y[g] = 1.125 * double(row) / double(s);
+    z[g] = 1.125 * double(plane) / double(s);
+}
+
+// Element volumes = reference volumes = masses (lulesh-init.cc:148-160),
+// v = 1, and the single-rank boundary flags (lulesh-init.cc:548-607).
+__global__ void k_sedov_elems(int32_t s, const double *__restrict__ x,
+                              const double *__restrict__ y, const double *__restrict__ z,
+                              double *__restrict__ volo, double *__restrict__ elemMass,
+                              double *__restrict__ v, int32_t *__restrict__ elemBC) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= s * s * s) return;
+    const int n = s + 1;
+    const int i = e % s, j = (e / s) % s, k = e / (s * s);
+    const int b = (k * n + j) * n + i;
+    const int nd[8] = {b, b + 1, b + n + 1, b + n, b + n * n, b + n * n + 1,
+                       b + n * n + n + 1, b + n * n + n};
+    double xl[8], yl[8], zl[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        xl[a] = x[nd[a]];
+        yl[a] = y[nd[a]];
+        zl[a] = z[nd[a]];
+    }
+    const double vol = elem_volume<double>(xl, yl, zl);
+    volo[e] = vol;
+    elemMass[e] = vol;
+    v[e] = 1.0;
+    int32_t bc = 0;
+    if (k == 0) bc |= ZETA_M_SYMM;
+    if (k == s - 1) bc |= ZETA_P_FREE;
+    if (j == 0) bc |= ETA_M_SYMM;
+    if (j == s - 1) bc |= ETA_P_FREE;
+    if (i == 0) bc |= XI_M_SYMM;
+    if (i == s - 1) bc |= XI_P_FREE;
+    elemBC[e] = bc;
+}
+
+// Nodal mass = sum of volume/8 over the node's elements in ascending
+// element order, starting from 0 (the constructor's element-by-element
+// accumulation, lulesh-init.cc:161-166).
+__global__ void k_sedov_nodal_mass(int32_t s, const double *__restrict__ volo,
+                                   double *__restrict__ nodalMass) {
+    const int n = s + 1;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n * n * n) return;
+    const int I = g % n, J = (g / n) % n, K = g / (n * n);
+    double m = 0.0;
+    for (int k = K - 1; k <= K; ++k)
+        for (int j = J - 1; j <= J; ++j)
+            for (int i = I - 1; i <= I; ++i)
+                if (k >= 0 && k < s && j >= 0 && j < s && i >= 0 && i < s)
+                    m += volo[(k * s + j) * s + i] / 8.0;
+    nodalMass[g] = m;
+}
+
+// EOS repetitions per element from the region runs: run r covers elements
+// [start[r], start[r+1]) of region region[r] (1-based), lulesh.cc:2159-2169.
+__global__ void k_sedov_reps(int32_t E, int32_t nruns, const int32_t *__restrict__ start,
+                             const int32_t *__restrict__ region, int32_t nr, int32_t cost,
+                             uint16_t *__restrict__ rep) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    int lo = 0, hi = nruns - 1; // last run with start <= e
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    rep[e] = uint16_t(region_rep(region[lo] - 1, nr, cost));
+}
+
 extern "C" {
 
 int lulesh_hip_create(const lulesh_domain_view *v, int precision_opts, lulesh_ctx **out) {
@@ -1571,6 +1658,141 @@ int lulesh_hip_create(const lulesh_domain_view *v, int precision_opts, lulesh_ct
 int lulesh_hip_batch_create(const lulesh_domain_view *const *views, int32_t nprob,
                             int precision_opts, lulesh_ctx **out) {
     return create_impl(views, nprob, precision_opts, out);
+}
+
+int lulesh_hip_create_sedov(int32_t s, int32_t nr, int32_t balance, int32_t cost,
+                            int precision_opts, lulesh_ctx **out) {
+    if (!out) return LULESH_INVALID_ARGUMENT;
+    *out = nullptr;
+    const int precision = precision_opts & 0xff;
+    if ((precision_opts & ~(0xff | LULESH_OPT_NODE_FORCES)) ||
+        (precision != LULESH_PRECISION_FP64 && precision != LULESH_PRECISION_MIXED) || s < 1 ||
+        nr < 1 || balance < 0 || cost < 0)
+        return LULESH_INVALID_ARGUMENT;
+    const int64_t E64 = int64_t(s) * s * s, N64 = int64_t(s + 1) * (s + 1) * (s + 1);
+    if (E64 * 8 >= (int64_t(1) << 31)) return LULESH_INVALID_ARGUMENT;
+    for (int32_t r = 0; r < nr; ++r) {
+        const int32_t rv = region_rep(r, nr, cost);
+        if (rv < 1 || rv > 65535) return LULESH_INVALID_ARGUMENT;
+    }
+    auto *c = new lulesh_ctx();
+    auto fail = [&](int rc) {
+        lulesh_hip_destroy(c);
+        return rc;
+    };
+    using D = lulesh::Domain;
+    c->precision = precision;
+    c->store_forces = (precision_opts & LULESH_OPT_NODE_FORCES) != 0;
+    c->structured = true;
+    c->nprob = 1;
+    c->totE = E64;
+    c->totN = N64;
+    c->P = Params{D::m_e_cut, D::m_p_cut, D::m_q_cut, D::m_v_cut, D::m_u_cut, D::m_hgcoef,
+                  D::m_monoq_max_slope, D::m_monoq_limiter_mult, D::m_qlc_monoq,
+                  D::m_qqc_monoq, D::m_qqc, D::m_eosvmax, D::m_eosvmin, D::m_pmin, D::m_emin,
+                  D::m_dvovmax, D::m_refdens};
+    c->hgcoef = D::m_hgcoef;
+    c->timers = {{"time_increment"}, {"nodal"}, {"nodes"}, {"elements"}, {"material"},
+                 {"minima"}};
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->run0) != hipSuccess || hipEventCreate(&c->run1) != hipSuccess)
+        return fail(LULESH_HIP_ERROR);
+    ProbHost ph;
+    ph.s = s;
+    ph.E = int32_t(E64);
+    ph.N = int32_t(N64);
+    ph.numReg = nr;
+    ph.cost = cost;
+    ph.elemOff = 0;
+    ph.nodeOff = 0;
+    c->probs.push_back(ph);
+    const size_t E = size_t(E64), N = size_t(N64);
+    int rc;
+    // every field zero-filled (the constructor's allocations), then set
+    for (int id = 0; id < LULESH_F_COUNT; ++id) {
+        const size_t n = is_node_field(id) ? N : E;
+        if (precision == LULESH_PRECISION_MIXED && is_grad_field(id)) {
+            if ((rc = dalloc(c, &c->g32[grad_slot(id)], n))) return fail(rc);
+            continue;
+        }
+        if ((rc = dalloc(c, &c->f[id], n))) return fail(rc);
+    }
+    for (int k = 0; k < 6; ++k) {
+        c->nb.b[0][k] = c->f[LULESH_F_X + k];
+        if ((rc = dalloc(c, &c->nb.b[1][k], N))) return fail(rc);
+    }
+    if ((rc = dalloc(c, &c->elemBC, E)) || (rc = dalloc(c, &c->erep, E))) return fail(rc);
+    {
+        const size_t ns = size_t(kMinSlots) * kSlotU64;
+        std::vector<unsigned long long> init(ns, kMinInit);
+        if ((rc = dalloc(c, &c->minslots, ns)) || (rc = upload(c, c->minslots, init.data(), ns)))
+            return fail(rc);
+    }
+    // region runs from the reference's rand() stream (O(runs) on the host)
+    std::vector<lulesh::RegionRun> runs;
+    lulesh::SedovRegionRuns(int32_t(E64), nr, balance, runs);
+    std::vector<int32_t> rstart(runs.size()), rregion(runs.size());
+    for (size_t r = 0; r < runs.size(); ++r) {
+        rstart[r] = runs[r].start;
+        rregion[r] = runs[r].region;
+    }
+    int32_t *d_rs = nullptr, *d_rr = nullptr;
+    if ((rc = dalloc(c, &d_rs, runs.size())) || (rc = upload(c, d_rs, rstart.data(), runs.size())) ||
+        (rc = dalloc(c, &d_rr, runs.size())) || (rc = upload(c, d_rr, rregion.data(), runs.size())))
+        return fail(rc);
+    // the zero fills and uploads above are complete before the set-up kernels
+    if (hipDeviceSynchronize() != hipSuccess) return fail(LULESH_HIP_ERROR);
+    const int gN = grid_for(int64_t(N), 256), gE = grid_for(int64_t(E), 256);
+    k_sedov_nodes<<<gN, 256, 0, c->stream>>>(s, c->f[LULESH_F_X], c->f[LULESH_F_Y],
+                                               c->f[LULESH_F_Z]);
+    k_sedov_elems<<<gE, 256, 0, c->stream>>>(s, c->f[LULESH_F_X], c->f[LULESH_F_Y],
+                                               c->f[LULESH_F_Z], c->f[LULESH_F_VOLO],
+                                               c->f[LULESH_F_ELEMMASS], c->f[LULESH_F_V],
+                                               c->elemBC);
+    k_sedov_nodal_mass<<<gN, 256, 0, c->stream>>>(s, c->f[LULESH_F_VOLO],
+                                                    c->f[LULESH_F_NODALMASS]);
+    k_sedov_reps<<<gE, 256, 0, c->stream>>>(int32_t(E64), int32_t(runs.size()), d_rs, d_rr, nr,
+                                              cost, c->erep);
+    HIP_TRY(c, hipGetLastError());
+    const double einit = lulesh::SedovInitialEnergy(s);
+    if ((rc = upload(c, c->f[LULESH_F_E], &einit, 1))) return fail(rc);
+    if ((rc = dalloc(c, &c->st, 1))) return fail(rc);
+    if (hipHostMalloc(reinterpret_cast<void **>(&c->hst), sizeof(DevState)) != hipSuccess)
+        return fail(LULESH_HIP_ERROR);
+    // run defaults, lulesh-init.cc:135-145
+    DevState st{};
+    st.time = 0.0;
+    st.deltatime = lulesh::SedovInitialDeltaTime(s);
+    st.stoptime = 1.0e-2;
+    st.dtfixed = -1.0e-6;
+    st.deltatimemultlb = 1.1;
+    st.deltatimemultub = 1.2;
+    st.dtmax = 1.0e-2;
+    st.dtc_bits = host_dbits(1.0e+20);
+    st.dth_bits = host_dbits(1.0e+20);
+    c->init.push_back(st);
+    if (hipMemcpy(c->st, &st, sizeof(DevState), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(LULESH_HIP_ERROR);
+    if ((rc = configure(c))) return fail(rc);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(LULESH_HIP_ERROR);
+    *out = c;
+    return 0;
+}
+
+int lulesh_hip_get_time_state(lulesh_ctx *c, lulesh_time_state *ts) {
+    if (!c || !ts || c->nprob != 1) return LULESH_INVALID_ARGUMENT;
+    if (int rc = read_states(c)) return rc;
+    to_time_state(c->hst[0], ts);
+    return 0;
+}
+
+int lulesh_hip_report(lulesh_ctx *c, double out[4]) {
+    if (!c || !out || c->nprob != 1) return LULESH_INVALID_ARGUMENT;
+    const int32_t nx = c->probs[0].s;
+    std::vector<double> e(size_t(nx) * nx);
+    if (int rc = field_copy(c, 0, LULESH_F_E, e.data(), e.size(), true)) return rc;
+    lulesh::ReportNumbers(e.data(), nx, out);
+    return 0;
 }
 
 int32_t lulesh_hip_num_problems(const lulesh_ctx *c) { return c ? c->nprob : 0; }

@@ -50,25 +50,28 @@ int LagrangeLeapFrog(Domain &domain, lulesh_ctx *ctx) {
     return rc;
 }
 
-void VerifyNumbers(Domain &domain, Real_t out[4]) {
+void ReportNumbers(const Real_t *e, Index_t nx, Real_t out[4]) {
     // Plane-0 symmetry check of the energy field (lulesh-util.cc:81-97): the
     // Sedov solution is symmetric under x <-> y, so e(j*nx+k) must equal
     // e(k*nx+j).  RelDiff divides by the signed e(k*nx+j).
-    const Index_t nx = domain.sizeX();
     Real_t maxAbs = 0.0, totalAbs = 0.0, maxRel = 0.0;
     for (Index_t j = 0; j < nx; ++j)
         for (Index_t k = j + 1; k < nx; ++k) {
-            const Real_t a = domain.e(j * nx + k), b = domain.e(k * nx + j);
+            const Real_t a = e[j * nx + k], b = e[k * nx + j];
             const Real_t absDiff = std::fabs(a - b);
             totalAbs += absDiff;
             if (maxAbs < absDiff) maxAbs = absDiff;
             const Real_t rel = std::fabs(b) > 1e-8 ? absDiff / b : 0.0;
             if (maxRel < rel) maxRel = rel;
         }
-    out[0] = domain.e(0);
+    out[0] = e[0];
     out[1] = maxAbs;
     out[2] = totalAbs;
     out[3] = maxRel;
+}
+
+void VerifyNumbers(Domain &domain, Real_t out[4]) {
+    ReportNumbers(&domain.e(0), domain.sizeX(), out);
 }
 
 Int_t RegionRep(Int_t reg, Int_t numReg, Int_t cost) {

@@ -111,19 +111,38 @@ Domain::Domain(Index_t nx, Int_t nr, Int_t balance, Int_t cost) {
     }
 
     // Energy deposit scaled by (nx/45)^3 into element 0, lulesh-init.cc:172-179
-    const Real_t ebase = 3.948746e+7;
-    const Real_t scale = nx / Real_t(45.0);
-    const Real_t einit = ebase * scale * scale * scale;
+    const Real_t einit = SedovInitialEnergy(nx);
     m_e[0] = einit;
     // Initial time step from the CFL estimate, lulesh-init.cc:181
     m_deltatime = (Real_t(.5) * std::cbrt(m_volo[0])) / std::sqrt(Real_t(2.0) * einit);
+}
+
+Real_t SedovInitialEnergy(Index_t nx) {
+    const Real_t ebase = 3.948746e+7;
+    const Real_t scale = nx / Real_t(45.0);
+    return ebase * scale * scale * scale;
+}
+
+Real_t SedovInitialDeltaTime(Index_t nx) {
+    // volo of element 0 from its lattice nodes, exactly as the constructor
+    // computes it
+    const Index_t n[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                             {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    Real_t x[8], y[8], z[8];
+    for (int k = 0; k < 8; ++k) {
+        x[k] = SedovCoord(n[k][0], nx);
+        y[k] = SedovCoord(n[k][1], nx);
+        z[k] = SedovCoord(n[k][2], nx);
+    }
+    const Real_t volo0 = CalcElemVolume(x, y, z);
+    return (Real_t(.5) * std::cbrt(volo0)) / std::sqrt(Real_t(2.0) * SedovInitialEnergy(nx));
 }
 
 void Domain::BuildMesh(Int_t nx) {
     // Lattice over [0, 1.125]^3; every coordinate is computed from its index
     // (lulesh-init.cc:207-227), never accumulated.
     const Index_t edgeNodes = nx + 1;
-    auto coord = [nx](Index_t i) { return Real_t(1.125) * Real_t(i) / Real_t(nx); };
+    auto coord = [nx](Index_t i) { return SedovCoord(i, nx); };
     Index_t nidx = 0;
     for (Index_t plane = 0; plane < edgeNodes; ++plane)
         for (Index_t row = 0; row < edgeNodes; ++row)
@@ -170,50 +189,60 @@ void Domain::SetupThreadSupportStructures() {
         }
 }
 
-void Domain::CreateRegionIndexSets(Int_t nr, Int_t balance) {
+void SedovRegionRuns(Index_t E, Int_t nr, Int_t balance, std::vector<RegionRun> &runs) {
     // Random region runs (lulesh-init.cc:341-445).  The reference draws from
     // libc rand() after srand(0); the same calls in the same order reproduce
     // its region map exactly.
+    runs.clear();
     std::srand(0);
+    if (nr == 1) {
+        runs.push_back({0, 1});
+        return;
+    }
+    // Cumulative weights (i+1)^balance; the running sum is kept in an int,
+    // as in the reference.
+    std::vector<Int_t> binEnd(static_cast<size_t>(nr));
+    Int_t denom = 0;
+    for (Int_t i = 0; i < nr; ++i) {
+        denom = Int_t(double(denom) + std::pow(double(i + 1), double(balance)));
+        binEnd[size_t(i)] = denom;
+    }
+    auto pick = [&]() {
+        const Int_t r = std::rand() % denom;
+        Int_t i = 0;
+        while (r >= binEnd[size_t(i)]) ++i;
+        return (i % nr) + 1;
+    };
+    Int_t lastReg = -1;
+    Index_t next = 0;
+    while (next < E) {
+        Int_t regionNum = pick();
+        while (regionNum == lastReg) regionNum = pick();
+        // Run length: a skewed distribution of bin sizes.
+        const Int_t bin = std::rand() % 1000;
+        Index_t run;
+        if (bin < 773) run = std::rand() % 15 + 1;
+        else if (bin < 937) run = std::rand() % 16 + 16;
+        else if (bin < 970) run = std::rand() % 32 + 32;
+        else if (bin < 974) run = std::rand() % 64 + 64;
+        else if (bin < 978) run = std::rand() % 128 + 128;
+        else if (bin < 981) run = std::rand() % 256 + 256;
+        else run = std::rand() % 1537 + 512;
+        runs.push_back({next, regionNum});
+        next += run;
+        lastReg = regionNum;
+    }
+}
+
+void Domain::CreateRegionIndexSets(Int_t nr, Int_t balance) {
     m_numReg = nr;
     m_regElemSize.assign(size_t(nr), 0);
     const Index_t E = m_numElem;
-    if (nr == 1) {
-        for (Index_t i = 0; i < E; ++i) m_regNumList[i] = 1;
-    } else {
-        // Cumulative weights (i+1)^balance; the running sum is kept in an
-        // int, as in the reference.
-        std::vector<Int_t> binEnd(static_cast<size_t>(nr));
-        Int_t denom = 0;
-        for (Int_t i = 0; i < nr; ++i) {
-            denom = Int_t(double(denom) + std::pow(double(i + 1), double(balance)));
-            binEnd[size_t(i)] = denom;
-        }
-        auto pick = [&]() {
-            const Int_t r = std::rand() % denom;
-            Int_t i = 0;
-            while (r >= binEnd[size_t(i)]) ++i;
-            return (i % nr) + 1;
-        };
-        Int_t lastReg = -1;
-        Index_t next = 0;
-        while (next < E) {
-            Int_t regionNum = pick();
-            while (regionNum == lastReg) regionNum = pick();
-            // Run length: a skewed distribution of bin sizes.
-            const Int_t bin = std::rand() % 1000;
-            Index_t run;
-            if (bin < 773) run = std::rand() % 15 + 1;
-            else if (bin < 937) run = std::rand() % 16 + 16;
-            else if (bin < 970) run = std::rand() % 32 + 32;
-            else if (bin < 974) run = std::rand() % 64 + 64;
-            else if (bin < 978) run = std::rand() % 128 + 128;
-            else if (bin < 981) run = std::rand() % 256 + 256;
-            else run = std::rand() % 1537 + 512;
-            const Index_t end = next + run;
-            for (; next < end && next < E; ++next) m_regNumList[next] = regionNum;
-            lastReg = regionNum;
-        }
+    std::vector<RegionRun> runs;
+    SedovRegionRuns(E, nr, balance, runs);
+    for (size_t r = 0; r < runs.size(); ++r) {
+        const Index_t end = r + 1 < runs.size() ? runs[r + 1].start : E;
+        for (Index_t i = runs[r].start; i < end && i < E; ++i) m_regNumList[i] = runs[r].region;
     }
     // Region index sets in ascending element order.
     for (Index_t i = 0; i < E; ++i) ++m_regElemSize[size_t(m_regNumList[i] - 1)];

@@ -160,14 +160,17 @@ class Domain {
         m_arealg, m_ss, m_elemMass;
     std::vector<Index_t> m_nodeElemStart, m_nodeElemCornerList;
 
-    // Constants, lulesh-init.cc:18-36
-    const Real_t m_e_cut = 1.0e-7, m_p_cut = 1.0e-7, m_q_cut = 1.0e-7, m_v_cut = 1.0e-10,
-                 m_u_cut = 1.0e-7;
-    const Real_t m_hgcoef = 3.0, m_ss4o3 = 4.0 / 3.0, m_qstop = 1.0e+12;
-    const Real_t m_monoq_max_slope = 1.0, m_monoq_limiter_mult = 2.0, m_qlc_monoq = 0.5,
-                 m_qqc_monoq = 2.0 / 3.0, m_qqc = 2.0;
-    const Real_t m_eosvmax = 1.0e+9, m_eosvmin = 1.0e-9, m_pmin = 0., m_emin = -1.0e+15,
-                 m_dvovmax = 0.1, m_refdens = 1.0;
+  public:
+    // Constants, lulesh-init.cc:18-36 (the same for every Domain)
+    static constexpr Real_t m_e_cut = 1.0e-7, m_p_cut = 1.0e-7, m_q_cut = 1.0e-7,
+                            m_v_cut = 1.0e-10, m_u_cut = 1.0e-7;
+    static constexpr Real_t m_hgcoef = 3.0, m_ss4o3 = 4.0 / 3.0, m_qstop = 1.0e+12;
+    static constexpr Real_t m_monoq_max_slope = 1.0, m_monoq_limiter_mult = 2.0,
+                            m_qlc_monoq = 0.5, m_qqc_monoq = 2.0 / 3.0, m_qqc = 2.0;
+    static constexpr Real_t m_eosvmax = 1.0e+9, m_eosvmin = 1.0e-9, m_pmin = 0.,
+                            m_emin = -1.0e+15, m_dvovmax = 0.1, m_refdens = 1.0;
+
+  private:
 
     Real_t m_dtcourant = 0, m_dthydro = 0, m_dtfixed = 0, m_time = 0, m_deltatime = 0,
            m_deltatimemultlb = 0, m_deltatimemultub = 0, m_dtmax = 0, m_stoptime = 0;
@@ -178,6 +181,30 @@ class Domain {
 // Element volume (the reference's CalcElemVolume, lulesh.cc:592-674), used by
 // the Domain constructor for volo / elemMass / nodalMass.
 Real_t CalcElemVolume(const Real_t x[8], const Real_t y[8], const Real_t z[8]);
+
+// Lattice coordinate of node index i along an edge of nx elements,
+// lulesh-init.cc:207-227.
+inline Real_t SedovCoord(Index_t i, Index_t nx) { return Real_t(1.125) * Real_t(i) / Real_t(nx); }
+
+// The energy deposited in element 0 and the initial time step of the Sedov
+// problem of nx elements per edge (lulesh-init.cc:172-181).
+Real_t SedovInitialEnergy(Index_t nx);
+Real_t SedovInitialDeltaTime(Index_t nx);
+
+// One run of the region map: elements [start, next run's start) belong to
+// region `region` (1-based).  SedovRegionRuns replays the reference's libc
+// rand() stream (lulesh-init.cc:341-445) for E elements, nr regions and the
+// given balance; the last run ends at E.
+struct RegionRun {
+    Index_t start;
+    Int_t region;
+};
+void SedovRegionRuns(Index_t E, Int_t nr, Int_t balance, std::vector<RegionRun> &runs);
+
+// Report numbers of VerifyAndWriteFinalOutput (lulesh-util.cc:60-115) from
+// the first nx*nx entries of the energy field: e(0), MaxAbsDiff,
+// TotalAbsDiff, MaxRelDiff of the plane-0 x <-> y symmetry check.
+void ReportNumbers(const Real_t *e, Index_t nx, Real_t out[4]);
 
 // TimeIncrement, lulesh.cc:182-229.
 void TimeIncrement(Domain &domain);

@@ -116,6 +116,10 @@ def lib():
         "lulesh_hip_autotune": (ctypes.c_int, [vp, i32, ctypes.POINTER(Tuning), _D]),
         "lulesh_domain_scale_energy": (ctypes.c_int, [vp, dbl]),
         "lulesh_batch_params": (ctypes.c_int, [ctypes.c_uint64, i32, i32, i32, _I, _I, _D]),
+        "lulesh_hip_create_sedov": (ctypes.c_int, [i32, i32, i32, i32, ctypes.c_int,
+                                                   ctypes.POINTER(vp)]),
+        "lulesh_hip_get_time_state": (ctypes.c_int, [vp, ctypes.POINTER(TimeState)]),
+        "lulesh_hip_report": (ctypes.c_int, [vp, _D]),
     }
     for name, (res, args) in more.items():
         fn = getattr(L, name)
@@ -210,6 +214,33 @@ class Device:
             raise LuleshError(rc, "lulesh_hip_create")
         self.h = h
         self.precision = precision
+
+    @classmethod
+    def sedov(cls, s, nr=11, balance=1, cost=1, precision=FP64, opts=0):
+        """The Sedov problem built directly in HBM (lulesh_hip_create_sedov);
+        no host Domain."""
+        self = cls.__new__(cls)
+        self.L = lib()
+        self.dom = None
+        self.E, self.N = s ** 3, (s + 1) ** 3
+        h = ctypes.c_void_p()
+        rc = self.L.lulesh_hip_create_sedov(s, nr, balance, cost, precision | opts,
+                                             ctypes.byref(h))
+        if rc != 0:
+            raise LuleshError(rc, "lulesh_hip_create_sedov")
+        self.h = h
+        self.precision = precision
+        return self
+
+    def time_state(self):
+        ts = TimeState()
+        self._check(self.L.lulesh_hip_get_time_state(self.h, ctypes.byref(ts)), "time_state")
+        return ts
+
+    def report(self):
+        out = np.zeros(4)
+        self._check(self.L.lulesh_hip_report(self.h, out.ctypes.data_as(_D)), "report")
+        return out
 
     def close(self):
         if getattr(self, "h", None):

@@ -102,6 +102,14 @@ def test_create_rejects_bad_views_without_device(pyl):
     assert L.lulesh_domain_create(0, 11, 1, 1) is None
     assert L.lulesh_hip_leapfrog(None, 1e-6, None, None) == pyl.INVALID_ARGUMENT
     assert L.lulesh_hip_run(None, 10, None) == pyl.INVALID_ARGUMENT
+    # device-side set-up: argument checks come before any device call
+    for args in [(0, 11, 1, 1, 0), (10, 0, 1, 1, 0), (10, 11, -1, 1, 0), (10, 11, 1, -1, 0),
+                 (10, 11, 1, 1, pyl.OPT_GENERIC), (10, 11, 1, 1, 7), (700, 11, 1, 1, 0),
+                 (10, 21, 1, 100000, 0)]:
+        assert L.lulesh_hip_create_sedov(*args, ctypes.byref(h)) == pyl.INVALID_ARGUMENT, args
+    assert L.lulesh_hip_create_sedov(10, 11, 1, 1, 0, None) == pyl.INVALID_ARGUMENT
+    assert L.lulesh_hip_get_time_state(None, None) == pyl.INVALID_ARGUMENT
+    assert L.lulesh_hip_report(None, None) == pyl.INVALID_ARGUMENT
 
 
 def test_driver_flag_errors():
@@ -112,3 +120,6 @@ def test_driver_flag_errors():
     assert r.returncode == 255 and "usage" in r.stderr
     r = subprocess.run([exe, "--precision", "fp8"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 255
+    r = subprocess.run([exe, "--device-setup", "--host-loop"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 255 and "host set-up" in r.stderr
