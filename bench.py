@@ -123,8 +123,8 @@ def batch_bench(pyl, nprob, its, seed, tune_cycles, precision):
     region counts in {11, 16, 21} and energy factors in [0.5, 2), each run
     for `its` cycles (or to its stop time) in one context."""
     sizes, regions, factors = pyl.batch_params(seed, nprob)
-    doms = [pyl.make_domain(int(s), int(r), float(f)) for s, r, f in zip(sizes, regions, factors)]
-    b = pyl.Batch(doms, precision)
+    # built in HBM: bitwise the batch of host Domains (tests/test_gpu_setup.py)
+    b = pyl.Batch.sedov(sizes, regions, factors, precision=precision)
     tuning = None
     if tune_cycles:
         t, _ = b.autotune(tune_cycles)

@@ -201,6 +201,13 @@ int lulesh_hip_autotune(lulesh_ctx *ctx, int32_t cycles, lulesh_tuning *best,
  * state starts at the constructor's (lulesh_hip_get_time_state). */
 int lulesh_hip_create_sedov(int32_t s, int32_t nr, int32_t balance, int32_t cost,
                             int precision_opts, lulesh_ctx **out);
+/* The batched ensemble built the same way: problem p has sizes[p] elements
+ * per edge, regions[p] regions and its deposit scaled by factors[p] > 0
+ * (as lulesh_domain_scale_energy scales a host Domain); equal, bit for bit,
+ * to lulesh_hip_batch_create on the corresponding host Domains. */
+int lulesh_hip_batch_create_sedov(const int32_t *sizes, const int32_t *regions,
+                                  const double *factors, int32_t nprob, int32_t balance,
+                                  int32_t cost, int precision_opts, lulesh_ctx **out);
 /* Time state of a single-problem context (time, dt, dtcourant, dthydro,
  * cycle), as lulesh_hip_run would hand it back. */
 int lulesh_hip_get_time_state(lulesh_ctx *ctx, lulesh_time_state *ts);

@@ -1648,32 +1648,31 @@ __global__ void k_sedov_reps(int32_t E, int32_t nruns, const int32_t *__restrict
     rep[e] = uint16_t(region_rep(region[lo] - 1, nr, cost));
 }
 
-extern "C" {
-
-int lulesh_hip_create(const lulesh_domain_view *v, int precision_opts, lulesh_ctx **out) {
-    const lulesh_domain_view *views[1] = {v};
-    return create_impl(views, 1, precision_opts, out);
-}
-
-int lulesh_hip_batch_create(const lulesh_domain_view *const *views, int32_t nprob,
-                            int precision_opts, lulesh_ctx **out) {
-    return create_impl(views, nprob, precision_opts, out);
-}
-
-int lulesh_hip_create_sedov(int32_t s, int32_t nr, int32_t balance, int32_t cost,
-                            int precision_opts, lulesh_ctx **out) {
+// Set-up in HBM of one or more Sedov problems (sizes[p], regions[p], the
+// deposit of element 0 scaled by factors[p] as lulesh_domain_scale_energy
+// does), concatenated like a batch of host Domains.
+int create_sedov_impl(const int32_t *sizes, const int32_t *regions, const double *factors,
+                      int32_t nprob, int32_t balance, int32_t cost, int precision_opts,
+                      lulesh_ctx **out) {
     if (!out) return LULESH_INVALID_ARGUMENT;
     *out = nullptr;
     const int precision = precision_opts & 0xff;
     if ((precision_opts & ~(0xff | LULESH_OPT_NODE_FORCES)) ||
-        (precision != LULESH_PRECISION_FP64 && precision != LULESH_PRECISION_MIXED) || s < 1 ||
-        nr < 1 || balance < 0 || cost < 0)
+        (precision != LULESH_PRECISION_FP64 && precision != LULESH_PRECISION_MIXED) ||
+        nprob < 1 || !sizes || !regions || !factors || balance < 0 || cost < 0)
         return LULESH_INVALID_ARGUMENT;
-    const int64_t E64 = int64_t(s) * s * s, N64 = int64_t(s + 1) * (s + 1) * (s + 1);
-    if (E64 * 8 >= (int64_t(1) << 31)) return LULESH_INVALID_ARGUMENT;
-    for (int32_t r = 0; r < nr; ++r) {
-        const int32_t rv = region_rep(r, nr, cost);
-        if (rv < 1 || rv > 65535) return LULESH_INVALID_ARGUMENT;
+    int64_t totE = 0, totN = 0;
+    for (int pi = 0; pi < nprob; ++pi) {
+        const int32_t s = sizes[pi], nr = regions[pi];
+        if (s < 1 || nr < 1 || !(factors[pi] > 0)) return LULESH_INVALID_ARGUMENT;
+        for (int32_t r = 0; r < nr; ++r) {
+            const int32_t rv = region_rep(r, nr, cost);
+            if (rv < 1 || rv > 65535) return LULESH_INVALID_ARGUMENT;
+        }
+        totE += int64_t(s) * s * s;
+        totN += int64_t(s + 1) * (s + 1) * (s + 1);
+        if (totE * 8 >= (int64_t(1) << 31) || totN >= (int64_t(1) << 31))
+            return LULESH_INVALID_ARGUMENT;
     }
     auto *c = new lulesh_ctx();
     auto fail = [&](int rc) {
@@ -1684,9 +1683,9 @@ int lulesh_hip_create_sedov(int32_t s, int32_t nr, int32_t balance, int32_t cost
     c->precision = precision;
     c->store_forces = (precision_opts & LULESH_OPT_NODE_FORCES) != 0;
     c->structured = true;
-    c->nprob = 1;
-    c->totE = E64;
-    c->totN = N64;
+    c->nprob = nprob;
+    c->totE = totE;
+    c->totN = totN;
     c->P = Params{D::m_e_cut, D::m_p_cut, D::m_q_cut, D::m_v_cut, D::m_u_cut, D::m_hgcoef,
                   D::m_monoq_max_slope, D::m_monoq_limiter_mult, D::m_qlc_monoq,
                   D::m_qqc_monoq, D::m_qqc, D::m_eosvmax, D::m_eosvmin, D::m_pmin, D::m_emin,
@@ -1697,16 +1696,23 @@ int lulesh_hip_create_sedov(int32_t s, int32_t nr, int32_t balance, int32_t cost
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->run0) != hipSuccess || hipEventCreate(&c->run1) != hipSuccess)
         return fail(LULESH_HIP_ERROR);
-    ProbHost ph;
-    ph.s = s;
-    ph.E = int32_t(E64);
-    ph.N = int32_t(N64);
-    ph.numReg = nr;
-    ph.cost = cost;
-    ph.elemOff = 0;
-    ph.nodeOff = 0;
-    c->probs.push_back(ph);
-    const size_t E = size_t(E64), N = size_t(N64);
+    {
+        int32_t eo = 0, no = 0;
+        for (int pi = 0; pi < nprob; ++pi) {
+            ProbHost ph;
+            ph.s = sizes[pi];
+            ph.E = ph.s * ph.s * ph.s;
+            ph.N = (ph.s + 1) * (ph.s + 1) * (ph.s + 1);
+            ph.numReg = regions[pi];
+            ph.cost = cost;
+            ph.elemOff = eo;
+            ph.nodeOff = no;
+            eo += ph.E;
+            no += ph.N;
+            c->probs.push_back(ph);
+        }
+    }
+    const size_t E = size_t(totE), N = size_t(totN);
     int rc;
     // every field zero-filled (the constructor's allocations), then set
     for (int id = 0; id < LULESH_F_COUNT; ++id) {
@@ -1723,60 +1729,106 @@ int lulesh_hip_create_sedov(int32_t s, int32_t nr, int32_t balance, int32_t cost
     }
     if ((rc = dalloc(c, &c->elemBC, E)) || (rc = dalloc(c, &c->erep, E))) return fail(rc);
     {
-        const size_t ns = size_t(kMinSlots) * kSlotU64;
+        const size_t ns = size_t(nprob) * kMinSlots * kSlotU64;
         std::vector<unsigned long long> init(ns, kMinInit);
         if ((rc = dalloc(c, &c->minslots, ns)) || (rc = upload(c, c->minslots, init.data(), ns)))
             return fail(rc);
     }
-    // region runs from the reference's rand() stream (O(runs) on the host)
-    std::vector<lulesh::RegionRun> runs;
-    lulesh::SedovRegionRuns(int32_t(E64), nr, balance, runs);
-    std::vector<int32_t> rstart(runs.size()), rregion(runs.size());
-    for (size_t r = 0; r < runs.size(); ++r) {
-        rstart[r] = runs[r].start;
-        rregion[r] = runs[r].region;
+    // region runs of every problem from the reference's rand() stream
+    // (O(runs) on the host), as concatenated element indices
+    std::vector<int32_t> rstart, rregion, rfirst{0};
+    {
+        std::vector<lulesh::RegionRun> runs;
+        for (int pi = 0; pi < nprob; ++pi) {
+            const ProbHost &ph = c->probs[size_t(pi)];
+            lulesh::SedovRegionRuns(ph.E, ph.numReg, balance, runs);
+            for (const auto &r : runs) {
+                rstart.push_back(r.start);
+                rregion.push_back(r.region);
+            }
+            rfirst.push_back(int32_t(rstart.size()));
+        }
     }
     int32_t *d_rs = nullptr, *d_rr = nullptr;
-    if ((rc = dalloc(c, &d_rs, runs.size())) || (rc = upload(c, d_rs, rstart.data(), runs.size())) ||
-        (rc = dalloc(c, &d_rr, runs.size())) || (rc = upload(c, d_rr, rregion.data(), runs.size())))
+    if ((rc = dalloc(c, &d_rs, rstart.size())) ||
+        (rc = upload(c, d_rs, rstart.data(), rstart.size())) ||
+        (rc = dalloc(c, &d_rr, rregion.size())) ||
+        (rc = upload(c, d_rr, rregion.data(), rregion.size())))
         return fail(rc);
     // the zero fills and uploads above are complete before the set-up kernels
     if (hipDeviceSynchronize() != hipSuccess) return fail(LULESH_HIP_ERROR);
-    const int gN = grid_for(int64_t(N), 256), gE = grid_for(int64_t(E), 256);
-    k_sedov_nodes<<<gN, 256, 0, c->stream>>>(s, c->f[LULESH_F_X], c->f[LULESH_F_Y],
-                                               c->f[LULESH_F_Z]);
-    k_sedov_elems<<<gE, 256, 0, c->stream>>>(s, c->f[LULESH_F_X], c->f[LULESH_F_Y],
-                                               c->f[LULESH_F_Z], c->f[LULESH_F_VOLO],
-                                               c->f[LULESH_F_ELEMMASS], c->f[LULESH_F_V],
-                                               c->elemBC);
-    k_sedov_nodal_mass<<<gN, 256, 0, c->stream>>>(s, c->f[LULESH_F_VOLO],
-                                                    c->f[LULESH_F_NODALMASS]);
-    k_sedov_reps<<<gE, 256, 0, c->stream>>>(int32_t(E64), int32_t(runs.size()), d_rs, d_rr, nr,
-                                              cost, c->erep);
-    HIP_TRY(c, hipGetLastError());
-    const double einit = lulesh::SedovInitialEnergy(s);
-    if ((rc = upload(c, c->f[LULESH_F_E], &einit, 1))) return fail(rc);
-    if ((rc = dalloc(c, &c->st, 1))) return fail(rc);
-    if (hipHostMalloc(reinterpret_cast<void **>(&c->hst), sizeof(DevState)) != hipSuccess)
+    for (int pi = 0; pi < nprob; ++pi) {
+        const ProbHost &ph = c->probs[size_t(pi)];
+        const size_t eo = size_t(ph.elemOff), no = size_t(ph.nodeOff);
+        const int gN = grid_for(ph.N, 256), gE = grid_for(ph.E, 256);
+        k_sedov_nodes<<<gN, 256, 0, c->stream>>>(ph.s, c->f[LULESH_F_X] + no,
+                                                   c->f[LULESH_F_Y] + no, c->f[LULESH_F_Z] + no);
+        k_sedov_elems<<<gE, 256, 0, c->stream>>>(
+            ph.s, c->f[LULESH_F_X] + no, c->f[LULESH_F_Y] + no, c->f[LULESH_F_Z] + no,
+            c->f[LULESH_F_VOLO] + eo, c->f[LULESH_F_ELEMMASS] + eo, c->f[LULESH_F_V] + eo,
+            c->elemBC + eo);
+        k_sedov_nodal_mass<<<gN, 256, 0, c->stream>>>(ph.s, c->f[LULESH_F_VOLO] + eo,
+                                                        c->f[LULESH_F_NODALMASS] + no);
+        k_sedov_reps<<<gE, 256, 0, c->stream>>>(
+            ph.E, rfirst[size_t(pi) + 1] - rfirst[size_t(pi)], d_rs + rfirst[size_t(pi)],
+            d_rr + rfirst[size_t(pi)], ph.numReg, cost, c->erep + eo);
+        HIP_TRY(c, hipGetLastError());
+    }
+    if ((rc = dalloc(c, &c->st, size_t(nprob)))) return fail(rc);
+    if (hipHostMalloc(reinterpret_cast<void **>(&c->hst), sizeof(DevState) * size_t(nprob)) !=
+        hipSuccess)
         return fail(LULESH_HIP_ERROR);
-    // run defaults, lulesh-init.cc:135-145
-    DevState st{};
-    st.time = 0.0;
-    st.deltatime = lulesh::SedovInitialDeltaTime(s);
-    st.stoptime = 1.0e-2;
-    st.dtfixed = -1.0e-6;
-    st.deltatimemultlb = 1.1;
-    st.deltatimemultub = 1.2;
-    st.dtmax = 1.0e-2;
-    st.dtc_bits = host_dbits(1.0e+20);
-    st.dth_bits = host_dbits(1.0e+20);
-    c->init.push_back(st);
-    if (hipMemcpy(c->st, &st, sizeof(DevState), hipMemcpyHostToDevice) != hipSuccess)
+    for (int pi = 0; pi < nprob; ++pi) {
+        const ProbHost &ph = c->probs[size_t(pi)];
+        // the deposit of element 0 and dt0, lulesh-init.cc:172-181 (scaled as
+        // lulesh_domain_scale_energy scales a host Domain)
+        const double einit = lulesh::SedovInitialEnergy(ph.s) * factors[pi];
+        if ((rc = upload(c, c->f[LULESH_F_E] + ph.elemOff, &einit, 1))) return fail(rc);
+        // run defaults, lulesh-init.cc:135-145
+        DevState st{};
+        st.time = 0.0;
+        st.deltatime = lulesh::SedovInitialDeltaTime(ph.s, einit);
+        st.stoptime = 1.0e-2;
+        st.dtfixed = -1.0e-6;
+        st.deltatimemultlb = 1.1;
+        st.deltatimemultub = 1.2;
+        st.dtmax = 1.0e-2;
+        st.dtc_bits = host_dbits(1.0e+20);
+        st.dth_bits = host_dbits(1.0e+20);
+        c->init.push_back(st);
+    }
+    if (hipMemcpy(c->st, c->init.data(), sizeof(DevState) * size_t(nprob),
+                  hipMemcpyHostToDevice) != hipSuccess)
         return fail(LULESH_HIP_ERROR);
     if ((rc = configure(c))) return fail(rc);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(LULESH_HIP_ERROR);
     *out = c;
     return 0;
+}
+
+extern "C" {
+
+int lulesh_hip_create(const lulesh_domain_view *v, int precision_opts, lulesh_ctx **out) {
+    const lulesh_domain_view *views[1] = {v};
+    return create_impl(views, 1, precision_opts, out);
+}
+
+int lulesh_hip_batch_create(const lulesh_domain_view *const *views, int32_t nprob,
+                            int precision_opts, lulesh_ctx **out) {
+    return create_impl(views, nprob, precision_opts, out);
+}
+
+int lulesh_hip_create_sedov(int32_t s, int32_t nr, int32_t balance, int32_t cost,
+                            int precision_opts, lulesh_ctx **out) {
+    const double one = 1.0;
+    return create_sedov_impl(&s, &nr, &one, 1, balance, cost, precision_opts, out);
+}
+
+int lulesh_hip_batch_create_sedov(const int32_t *sizes, const int32_t *regions,
+                                  const double *factors, int32_t nprob, int32_t balance,
+                                  int32_t cost, int precision_opts, lulesh_ctx **out) {
+    return create_sedov_impl(sizes, regions, factors, nprob, balance, cost, precision_opts,
+                             out);
 }
 
 int lulesh_hip_get_time_state(lulesh_ctx *c, lulesh_time_state *ts) {

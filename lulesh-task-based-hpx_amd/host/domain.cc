@@ -123,7 +123,7 @@ Real_t SedovInitialEnergy(Index_t nx) {
     return ebase * scale * scale * scale;
 }
 
-Real_t SedovInitialDeltaTime(Index_t nx) {
+Real_t SedovInitialDeltaTime(Index_t nx, Real_t einit) {
     // volo of element 0 from its lattice nodes, exactly as the constructor
     // computes it
     const Index_t n[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
@@ -135,7 +135,7 @@ Real_t SedovInitialDeltaTime(Index_t nx) {
         z[k] = SedovCoord(n[k][2], nx);
     }
     const Real_t volo0 = CalcElemVolume(x, y, z);
-    return (Real_t(.5) * std::cbrt(volo0)) / std::sqrt(Real_t(2.0) * SedovInitialEnergy(nx));
+    return (Real_t(.5) * std::cbrt(volo0)) / std::sqrt(Real_t(2.0) * einit);
 }
 
 void Domain::BuildMesh(Int_t nx) {

@@ -189,7 +189,9 @@ inline Real_t SedovCoord(Index_t i, Index_t nx) { return Real_t(1.125) * Real_t(
 // The energy deposited in element 0 and the initial time step of the Sedov
 // problem of nx elements per edge (lulesh-init.cc:172-181).
 Real_t SedovInitialEnergy(Index_t nx);
-Real_t SedovInitialDeltaTime(Index_t nx);
+// dt0 for a deposit einit in element 0 (the constructor's, lulesh-init.cc:181,
+// for einit = SedovInitialEnergy(nx))
+Real_t SedovInitialDeltaTime(Index_t nx, Real_t einit);
 
 // One run of the region map: elements [start, next run's start) belong to
 // region `region` (1-based).  SedovRegionRuns replays the reference's libc

@@ -119,6 +119,8 @@ def lib():
         "lulesh_hip_create_sedov": (ctypes.c_int, [i32, i32, i32, i32, ctypes.c_int,
                                                    ctypes.POINTER(vp)]),
         "lulesh_hip_get_time_state": (ctypes.c_int, [vp, ctypes.POINTER(TimeState)]),
+        "lulesh_hip_batch_create_sedov": (ctypes.c_int, [_I, _I, _D, i32, i32, i32, ctypes.c_int,
+                                                         ctypes.POINTER(vp)]),
         "lulesh_hip_report": (ctypes.c_int, [vp, _D]),
     }
     for name, (res, args) in more.items():
@@ -358,6 +360,27 @@ class Batch:
         if rc != 0:
             raise LuleshError(rc, "lulesh_hip_batch_create")
         self.h = h
+        self.sizes = [int(d.view.sizeX) for d in doms]
+
+    @classmethod
+    def sedov(cls, sizes, regions, factors, balance=1, cost=1, precision=FP64, opts=0):
+        """The ensemble built directly in HBM (lulesh_hip_batch_create_sedov)."""
+        self = cls.__new__(cls)
+        self.L = lib()
+        self.doms = None
+        self.sizes = [int(x) for x in sizes]
+        n = len(sizes)
+        s = np.ascontiguousarray(sizes, np.int32)
+        r = np.ascontiguousarray(regions, np.int32)
+        f = np.ascontiguousarray(factors, np.float64)
+        h = ctypes.c_void_p()
+        rc = self.L.lulesh_hip_batch_create_sedov(s.ctypes.data_as(_I), r.ctypes.data_as(_I),
+                                                   f.ctypes.data_as(_D), n, balance, cost,
+                                                   precision | opts, ctypes.byref(h))
+        if rc != 0:
+            raise LuleshError(rc, "lulesh_hip_batch_create_sedov")
+        self.h = h
+        return self
 
     def close(self):
         if getattr(self, "h", None):
@@ -368,13 +391,13 @@ class Batch:
         self.close()
 
     def run(self, max_cycles):
-        states = (TimeState * len(self.doms))()
+        states = (TimeState * len(self.sizes))()
         rc = self.L.lulesh_hip_batch_run(self.h, max_cycles, states)
         return rc, list(states)
 
     def download(self, prob, name):
-        d = self.doms[prob]
-        n = d.N if name in NODE_FIELDS else d.E
+        s = self.sizes[prob]
+        n = (s + 1) ** 3 if name in NODE_FIELDS else s ** 3
         out = np.empty(n, np.float64)
         rc = self.L.lulesh_hip_batch_download(self.h, prob, FIELD_ID[name], out.ctypes.data, n)
         if rc != 0:

@@ -68,3 +68,25 @@ def test_device_setup_s250(pyl):
         np.testing.assert_array_equal(dev.download(f), host.download(f), err_msg=f)
     dev.close()
     host.close()
+
+
+def test_batch_setup_in_hbm_equals_host_batch(pyl):
+    """The ensemble built in HBM equals the batch of host Domains, bit for
+    bit, before and after 60 cycles (per-problem dt and stop)."""
+    banner("batch of 12")
+    sizes, regions, factors = pyl.batch_params(3, 12, 5, 30)
+    doms = [pyl.make_domain(int(s), int(r), float(f)) for s, r, f in zip(sizes, regions, factors)]
+    host = pyl.Batch(doms)
+    dev = pyl.Batch.sedov(sizes, regions, factors)
+    for k, s in enumerate(sizes):
+        for f in FIELDS_ALL:
+            np.testing.assert_array_equal(dev.download(k, f), host.download(k, f),
+                                          err_msg=f"{k} {f}")
+    rh, sh = host.run(60)
+    rd, sd = dev.run(60)
+    assert rh == rd == 0
+    for k in range(len(sizes)):
+        assert state_tuple(sh[k]) == state_tuple(sd[k])
+        for f in ["x", "xd", "e", "p", "q", "v", "ss"]:
+            np.testing.assert_array_equal(dev.download(k, f), host.download(k, f),
+                                          err_msg=f"{k} {f}")
