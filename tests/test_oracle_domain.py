@@ -103,8 +103,9 @@ def test_initial_energy_and_dt():
 
 def test_region_sizes_s30():
     """Region sizes at -s 30 -r 11 -b 1 -c 1 (the region map comes from glibc
-    rand() after srand(0)); the list was derived independently of this
-    oracle when the reference was surveyed."""
+    rand() after srand(0)); the list is an independent derivation of the
+    reference's CreateRegionIndexSets (lulesh-init.cc:341-445), made apart
+    from this oracle."""
     d = OracleDomain(30)
     sizes = [len(d.region(r)) for r in range(d.num_reg)]
     assert sizes == [123, 457, 493, 2733, 1004, 896, 3236, 2435, 1198, 5980, 8445]
