@@ -171,6 +171,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-large", action="store_true", help="skip the s=250 line")
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--batch-its", type=int, default=1000)
     ap.add_argument("--tune-cycles", type=int, default=0,
@@ -259,6 +260,31 @@ def main():
         devm.close()
         domm.close()
 
+    # ---- the large single problem: s=250, 21 regions, 500 cycles ----------
+    large = None
+    if not args.no_large:
+        large = {"config": "s=250, 21 regions, -b 1 -c 1, 500 cycles from the start, "
+                           "built in HBM (bitwise the host set-up)"}
+        e0 = {}
+        for name, prec in (("fp64", pyl.FP64), ("mixed", pyl.MIXED)):
+            dl = pyl.Device.sedov(250, 21, precision=prec)
+            tl = dl.time_state()
+            rc = dl.run(10, tl)
+            assert rc == 0, rc
+            t0 = time.perf_counter()
+            rc = dl.run(500, tl)
+            el = time.perf_counter() - t0
+            assert rc == 0 and tl.cycle == 500, rc
+            rep = dl.report()
+            e0[name] = rep[0]
+            large[name] = {"value": 250 ** 3 * 490 / el, "ms_per_step": 1e3 * el / 490,
+                           "symmetry_max_rel_diff": float(rep[3])}
+            dl.close()
+        large["mixed"]["origin_energy_rel_err_vs_fp64"] = abs(e0["mixed"] - e0["fp64"]) / abs(
+            e0["fp64"])
+        print(f"[bench] s=250: fp64 {large['fp64']['value']:.4e} zc/s, "
+              f"mixed {large['mixed']['value']:.4e} zc/s", file=sys.stderr, flush=True)
+
     # ---- 64-problem batch -------------------------------------------------
     batch = None
     if not args.no_batch:
@@ -317,6 +343,7 @@ def main():
         "tuning": tuning,
         "mixed": mixed,
         "batch": batch,
+        "large": large,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (zcps / cpu["value"]) if cpu else None,
     }
