@@ -138,6 +138,27 @@ def test_device_loop_vs_oracle(pyl, s, its, path):
     assert rep[1] <= max(orep[1] * 10, 1e-9 * abs(orep[0]))
 
 
+@pytest.mark.parametrize("s,nr,b,c,its", [(12, 16, 2, 2, 150), (9, 1, 1, 1, 120),
+                                           (14, 21, 3, 0, 150)])
+def test_device_loop_vs_oracle_region_options(pyl, s, nr, b, c, its):
+    """Region count, balance (-b) and cost (-c) other than the defaults:
+    the region map, the repetition counts and the resulting cycle against
+    the oracle (device set-up and host set-up alike)."""
+    banner(f"device loop s={s} nr={nr} b={b} c={c}")
+    ora = OracleDomain(s, nr, b, c)
+    assert ora.run(its) == 0
+    dom, dev, ts = pyl.run_device(s, its, nr, b, c)
+    assert ts.cycle == ora.cycle == its
+    for f in STATE:
+        assert max_rel(dom.field(f), ora.f64(f)) < 1e-9, f
+    d2 = pyl.Device.sedov(s, nr, b, c)
+    t2 = d2.time_state()
+    assert d2.run(its, t2) == 0
+    for f in ("e", "p", "x", "xd"):
+        np.testing.assert_array_equal(d2.download(f), dom.field(f), err_msg=f)
+    d2.close()
+
+
 def test_host_loop_matches_device_loop(pyl):
     """The reference loop shape (host TimeIncrement + one leapfrog call per
     cycle) and the device-side loop give identical states; odd cycle counts
