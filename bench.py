@@ -23,6 +23,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "lulesh-task-based-hpx_amd")
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 
 
 def load_pylulesh():
@@ -70,8 +71,8 @@ PROF_CYCLES = 64  # cycles of the per-kernel event timing pass
 def measured(key, kernel):
     """A per-launch figure of `kernel` at s=150 FP64 from the committed
     rocprofv3 summary: key "traffic" = HBM bytes (FETCH_SIZE + WRITE_SIZE,
-    copy-calibrated), key "valu_busy" = fraction of time the VALUs issue.
-    None if the summary is absent."""
+    copy-calibrated), "valu_busy" = fraction of time the VALUs issue,
+    "fp64_flop" = FP64 lane operations (FMA = 2).  None if absent."""
     path = os.path.join(ROOT, TRAFFIC_FILE)
     if not os.path.exists(path):
         return None
@@ -239,6 +240,13 @@ def main():
     prof = (s, nr) == (150, 11)
     traffic = measured("traffic", dom_k) if prof else None
     valu_busy = measured("valu_busy", dom_k) if prof else None
+    fp64_flop = measured("fp64_flop", dom_k) if prof else None
+    fp64 = None
+    if fp64_flop:
+        tf = fp64_flop / (dk["avg_ms"] * 1e-3) / 1e12
+        fp64 = {"flop_per_launch": int(fp64_flop), "achieved": round(tf, 2),
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(tf / FP64_PEAK_TFLOPS, 4)}
     ms_cycle_events = sum(k["avg_ms"] * k["launches_per_cycle"] for k in kernels.values())
     dev.close()
 
@@ -330,6 +338,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": int(traffic) if traffic else None,
                      "valu_busy": valu_busy,
+                     "fp64_vector": fp64,
                      "source_of_traffic_and_valu_busy": TRAFFIC_FILE if traffic else None},
         "model_bytes_per_zone_cycle": round(bsz, 1),
         "model_hbm_frac": round(bsz * zcps / (HBM_PEAK_GBS * 1e9), 4),
