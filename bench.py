@@ -119,13 +119,15 @@ def timed_run(pyl, dev, dom, warmup, steps, tune_cycles):
     return t1 - t0, ts, tuning
 
 
-def batch_bench(pyl, nprob, its, seed, tune_cycles, precision):
+def batch_bench(pyl, nprob, its, seed, tune_cycles, precision, streams=1):
     """The ensemble: nprob Sedov problems with seeded sizes in [20, 80],
     region counts in {11, 16, 21} and energy factors in [0.5, 2), each run
-    for `its` cycles (or to its stop time) in one context."""
+    for `its` cycles (or to its stop time) in one context, split over
+    `streams` streams."""
     sizes, regions, factors = pyl.batch_params(seed, nprob)
     # built in HBM: bitwise the batch of host Domains (tests/test_gpu_setup.py)
-    b = pyl.Batch.sedov(sizes, regions, factors, precision=precision)
+    b = pyl.Batch.sedov(sizes, regions, factors, precision=precision,
+                        opts=pyl.OPT_STREAMS(streams) if streams > 1 else 0)
     tuning = None
     if tune_cycles:
         t, _ = b.autotune(tune_cycles)
@@ -140,6 +142,7 @@ def batch_bench(pyl, nprob, its, seed, tune_cycles, precision):
            "elapsed_s": el, "device_ms": b.last_run_ms(),
            "cycles_min": min(st.cycle for st in states),
            "cycles_max": max(st.cycle for st in states), "seed": seed, "tuning": tuning,
+           "streams": streams,
            "dtype": "f64" if precision == pyl.FP64 else "mixed"}
     b.close()
     return out
@@ -175,6 +178,8 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the s=250 line")
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--batch-its", type=int, default=1000)
+    ap.add_argument("--batch-streams", type=int, default=2,
+                    help="streams the batch is split over (1..15)")
     ap.add_argument("--tune-cycles", type=int, default=0,
                     help="autotune the s=150 legs first (0: the library's default launch shapes)")
     ap.add_argument("--batch-tune-cycles", type=int, default=6,
@@ -297,7 +302,8 @@ def main():
     batch = None
     if not args.no_batch:
         print(f"[bench] batch of {args.batch}", file=sys.stderr, flush=True)
-        batch = batch_bench(pyl, args.batch, args.batch_its, 0, args.batch_tune_cycles, pyl.FP64)
+        batch = batch_bench(pyl, args.batch, args.batch_its, 0, args.batch_tune_cycles, pyl.FP64,
+                            args.batch_streams)
         print(f"[bench] batch: {batch['value']:.4e} zc/s", file=sys.stderr, flush=True)
 
     # ---- CPU baseline ----------------------------------------------------

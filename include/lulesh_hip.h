@@ -49,6 +49,14 @@ extern "C" {
                                         (they are dead between cycles; off by default) */
 #define LULESH_OPT_GENERIC 0x200     /* use the indexed kernels (nodelist + corner CSR)
                                         even when the mesh is the structured Sedov cube */
+/* Batched ensembles only: split the problems over k streams (2 <= k <= 15,
+ * at most one per problem), each a complete context over its share (largest
+ * problems first, each to the stream with the fewest elements so far) driven
+ * by its own host thread in lulesh_hip_batch_run.  Kernels of one stream
+ * fill the gaps the other leaves at its launch and graph boundaries.  Every
+ * problem's fields and time state are bit-identical to the one-stream
+ * batch.  No counterpart in the reference, which runs one problem. */
+#define LULESH_OPT_STREAMS(k) (((k) & 0xf) << 12)
 
 /*
  * Plain-pointer view of a Domain (lulesh.h:130-606).  Array lengths:

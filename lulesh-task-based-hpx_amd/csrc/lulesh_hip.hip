@@ -30,9 +30,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -752,6 +754,11 @@ struct ProbHost {
 };
 
 struct lulesh_ctx {
+    // A batch split over several streams is a group: `lanes` are complete
+    // contexts over disjoint subsets of the problems (problem p is problem
+    // local_of[p] of lane lane_of[p]) and the group holds no device data.
+    std::vector<lulesh_ctx *> lanes;
+    std::vector<int32_t> lane_of, local_of;
     int precision = 0;
     bool structured = false;
     bool store_forces = false; // also write fx.. and xdd.. (Domain fidelity)
@@ -1511,7 +1518,8 @@ int create_impl(const lulesh_domain_view *const *views, int nprob, int precision
 // Host <-> device copy of one problem's field.  Node state fields live in
 // the half of the double buffer named by the problem's DevState.
 int field_copy(lulesh_ctx *c, int prob, int id, void *host, size_t count, bool down) {
-    if (!c || !host || id < 0 || id >= LULESH_F_COUNT || prob < 0 || prob >= c->nprob)
+    if (!c || !host || id < 0 || id >= LULESH_F_COUNT || prob < 0 || prob >= c->nprob ||
+        !c->lanes.empty())
         return LULESH_INVALID_ARGUMENT;
     const ProbHost &ph = c->probs[size_t(prob)];
     const size_t n = is_node_field(id) ? size_t(ph.N) : size_t(ph.E);
@@ -1806,6 +1814,62 @@ int create_sedov_impl(const int32_t *sizes, const int32_t *regions, const double
     return 0;
 }
 
+// Streams requested by the options (LULESH_OPT_STREAMS), at most one per
+// problem, 1 if none.
+int requested_streams(int precision_opts, int nprob) {
+    const int k = std::min((precision_opts >> 12) & 0xf, nprob);
+    return k < 1 ? 1 : k;
+}
+
+// Deal nprob problems of E[p] elements out over k lanes: largest first, each
+// to the lane with the fewest elements so far (ties: the lowest lane); each
+// lane keeps its problems in ascending order.
+std::vector<std::vector<int32_t>> deal_problems(const std::vector<int64_t> &E, int k) {
+    std::vector<int32_t> order(E.size());
+    for (size_t p = 0; p < E.size(); ++p) order[p] = int32_t(p);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return E[size_t(a)] > E[size_t(b)]; });
+    std::vector<std::vector<int32_t>> lanes(static_cast<size_t>(k));
+    std::vector<int64_t> load(static_cast<size_t>(k), 0);
+    for (int32_t p : order) {
+        const size_t l = size_t(std::min_element(load.begin(), load.end()) - load.begin());
+        lanes[l].push_back(p);
+        load[l] += E[size_t(p)];
+    }
+    for (auto &l : lanes) std::sort(l.begin(), l.end());
+    return lanes;
+}
+
+// A group over `parts` (lane -> global problem indices) whose lanes are made
+// by `make(lane, out)`.
+template <typename Make>
+int make_group(const std::vector<std::vector<int32_t>> &parts, int nprob, Make &&make,
+               lulesh_ctx **out) {
+    auto *g = new lulesh_ctx();
+    g->nprob = nprob;
+    g->lane_of.assign(size_t(nprob), 0);
+    g->local_of.assign(size_t(nprob), 0);
+    for (size_t l = 0; l < parts.size(); ++l) {
+        lulesh_ctx *lane = nullptr;
+        const int rc = make(l, &lane);
+        if (rc) {
+            lulesh_hip_destroy(g);
+            return rc;
+        }
+        g->lanes.push_back(lane);
+        g->bytes += lane->bytes;
+        for (size_t k = 0; k < parts[l].size(); ++k) {
+            g->lane_of[size_t(parts[l][k])] = int32_t(l);
+            g->local_of[size_t(parts[l][k])] = int32_t(k);
+        }
+    }
+    g->precision = g->lanes[0]->precision;
+    g->structured = g->lanes[0]->structured;
+    g->tune = g->lanes[0]->tune;
+    *out = g;
+    return 0;
+}
+
 extern "C" {
 
 int lulesh_hip_create(const lulesh_domain_view *v, int precision_opts, lulesh_ctx **out) {
@@ -1815,7 +1879,20 @@ int lulesh_hip_create(const lulesh_domain_view *v, int precision_opts, lulesh_ct
 
 int lulesh_hip_batch_create(const lulesh_domain_view *const *views, int32_t nprob,
                             int precision_opts, lulesh_ctx **out) {
-    return create_impl(views, nprob, precision_opts, out);
+    const int k = requested_streams(precision_opts, nprob);
+    const int opts = precision_opts & ~LULESH_OPT_STREAMS(0xf);
+    if (k == 1 || !views || !out) return create_impl(views, nprob, opts, out);
+    std::vector<int64_t> E(static_cast<size_t>(nprob));
+    for (int p = 0; p < nprob; ++p) {
+        if (!views[p]) return LULESH_INVALID_ARGUMENT;
+        E[size_t(p)] = views[p]->numElem;
+    }
+    const auto parts = deal_problems(E, k);
+    return make_group(parts, nprob, [&](size_t l, lulesh_ctx **lane) {
+        std::vector<const lulesh_domain_view *> sub;
+        for (int32_t p : parts[l]) sub.push_back(views[p]);
+        return create_impl(sub.data(), int(sub.size()), opts, lane);
+    }, out);
 }
 
 int lulesh_hip_create_sedov(int32_t s, int32_t nr, int32_t balance, int32_t cost,
@@ -1827,8 +1904,24 @@ int lulesh_hip_create_sedov(int32_t s, int32_t nr, int32_t balance, int32_t cost
 int lulesh_hip_batch_create_sedov(const int32_t *sizes, const int32_t *regions,
                                   const double *factors, int32_t nprob, int32_t balance,
                                   int32_t cost, int precision_opts, lulesh_ctx **out) {
-    return create_sedov_impl(sizes, regions, factors, nprob, balance, cost, precision_opts,
-                             out);
+    const int k = requested_streams(precision_opts, nprob);
+    const int opts = precision_opts & ~LULESH_OPT_STREAMS(0xf);
+    if (k == 1 || !sizes || !regions || !factors || !out)
+        return create_sedov_impl(sizes, regions, factors, nprob, balance, cost, opts, out);
+    std::vector<int64_t> E(static_cast<size_t>(nprob));
+    for (int p = 0; p < nprob; ++p) E[size_t(p)] = int64_t(sizes[p]) * sizes[p] * sizes[p];
+    const auto parts = deal_problems(E, k);
+    return make_group(parts, nprob, [&](size_t l, lulesh_ctx **lane) {
+        std::vector<int32_t> ss, rr;
+        std::vector<double> ff;
+        for (int32_t p : parts[l]) {
+            ss.push_back(sizes[p]);
+            rr.push_back(regions[p]);
+            ff.push_back(factors[p]);
+        }
+        return create_sedov_impl(ss.data(), rr.data(), ff.data(), int32_t(ss.size()), balance,
+                                 cost, opts, lane);
+    }, out);
 }
 
 int lulesh_hip_get_time_state(lulesh_ctx *c, lulesh_time_state *ts) {
@@ -1928,6 +2021,32 @@ int lulesh_hip_run(lulesh_ctx *c, int32_t max_cycles, lulesh_time_state *io) {
 
 int lulesh_hip_batch_run(lulesh_ctx *c, int32_t max_cycles, lulesh_time_state *states) {
     if (!c) return LULESH_INVALID_ARGUMENT;
+    if (!c->lanes.empty()) {
+        // every lane on its own stream, driven by its own host thread
+        const size_t L = c->lanes.size();
+        std::vector<std::vector<lulesh_time_state>> st(L);
+        std::vector<int> rcs(L, 0);
+        for (size_t l = 0; l < L; ++l) st[l].resize(size_t(c->lanes[l]->nprob));
+        const auto t0 = std::chrono::steady_clock::now();
+        std::vector<std::thread> th;
+        for (size_t l = 1; l < L; ++l)
+            th.emplace_back([&, l] { rcs[l] = lulesh_hip_batch_run(c->lanes[l], max_cycles,
+                                                                   st[l].data()); });
+        rcs[0] = lulesh_hip_batch_run(c->lanes[0], max_cycles, st[0].data());
+        for (auto &t : th) t.join();
+        c->last_run_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                .count();
+        if (states)
+            for (int p = 0; p < c->nprob; ++p)
+                states[p] = st[size_t(c->lane_of[size_t(p)])][size_t(c->local_of[size_t(p)])];
+        for (size_t l = 0; l < L; ++l)
+            if (rcs[l]) {
+                c->err = c->lanes[l]->err;
+                return rcs[l];
+            }
+        return 0;
+    }
     int rc = read_states(c);
     if (rc) return rc;
     for (int pi = 0; pi < c->nprob; ++pi) c->hst[pi].its = max_cycles;
@@ -1949,6 +2068,11 @@ int lulesh_hip_upload(lulesh_ctx *c, int id, const void *src, size_t count) {
 }
 
 int lulesh_hip_batch_download(lulesh_ctx *c, int32_t prob, int id, void *dst, size_t count) {
+    if (c && !c->lanes.empty()) {
+        if (prob < 0 || prob >= c->nprob) return LULESH_INVALID_ARGUMENT;
+        return field_copy(c->lanes[size_t(c->lane_of[size_t(prob)])],
+                          c->local_of[size_t(prob)], id, dst, count, true);
+    }
     return field_copy(c, prob, id, dst, count, true);
 }
 
@@ -1973,6 +2097,12 @@ int lulesh_hip_download_view(lulesh_ctx *c, lulesh_domain_view *dst) {
 
 int lulesh_hip_set_tuning(lulesh_ctx *c, const lulesh_tuning *t) {
     if (!c || !t) return LULESH_INVALID_ARGUMENT;
+    if (!c->lanes.empty()) {
+        for (lulesh_ctx *l : c->lanes)
+            if (int rc = lulesh_hip_set_tuning(l, t)) return rc;
+        c->tune = c->lanes[0]->tune;
+        return 0;
+    }
     c->tune = *t;
     return configure(c);
 }
@@ -1986,6 +2116,19 @@ int lulesh_hip_get_tuning(const lulesh_ctx *c, lulesh_tuning *t) {
 int lulesh_hip_autotune(lulesh_ctx *c, int32_t cycles, lulesh_tuning *best,
                         double *best_ms_per_cycle) {
     if (!c || cycles < 2) return LULESH_INVALID_ARGUMENT;
+    if (!c->lanes.empty()) {
+        // each lane on its own (the lanes are timed one after another)
+        double sum = 0;
+        for (lulesh_ctx *l : c->lanes) {
+            double ms = 0;
+            if (int rc = lulesh_hip_autotune(l, cycles, nullptr, &ms)) return rc;
+            sum += ms;
+        }
+        c->tune = c->lanes[0]->tune;
+        if (best) *best = c->tune;
+        if (best_ms_per_cycle) *best_ms_per_cycle = sum;
+        return 0;
+    }
     // Snapshot every state buffer; each candidate runs `cycles` cycles from
     // the snapshot and the state is restored afterwards.
     Snapshot snap;
@@ -2082,6 +2225,11 @@ int lulesh_hip_autotune(lulesh_ctx *c, int32_t cycles, lulesh_tuning *best,
 
 int lulesh_hip_set_timing(lulesh_ctx *c, int enable) {
     if (!c) return LULESH_INVALID_ARGUMENT;
+    if (!c->lanes.empty()) {
+        for (lulesh_ctx *l : c->lanes)
+            if (int rc = lulesh_hip_set_timing(l, enable)) return rc;
+        return 0;
+    }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->tev_used = 0;
     c->timing = enable != 0;
@@ -2095,6 +2243,27 @@ int lulesh_hip_set_timing(lulesh_ctx *c, int enable) {
 int lulesh_hip_kernel_times(lulesh_ctx *c, const char **names, double *avg_ms,
                             int64_t *launches, int cap) {
     if (!c) return LULESH_INVALID_ARGUMENT;
+    if (!c->lanes.empty()) {
+        // launches summed over the lanes, averages weighted by launches
+        std::vector<double> tot(size_t(std::max(cap, 0)), 0.0);
+        std::vector<int64_t> cnt(size_t(std::max(cap, 0)), 0);
+        int n = 0;
+        for (lulesh_ctx *l : c->lanes) {
+            std::vector<double> a(tot.size());
+            std::vector<int64_t> k(tot.size());
+            n = lulesh_hip_kernel_times(l, names, a.data(), k.data(), cap);
+            if (n < 0) return n;
+            for (int i = 0; i < n && i < cap; ++i) {
+                tot[size_t(i)] += a[size_t(i)] * double(k[size_t(i)]);
+                cnt[size_t(i)] += k[size_t(i)];
+            }
+        }
+        for (int i = 0; i < n && i < cap; ++i) {
+            if (avg_ms) avg_ms[i] = cnt[size_t(i)] ? tot[size_t(i)] / double(cnt[size_t(i)]) : 0.0;
+            if (launches) launches[i] = cnt[size_t(i)];
+        }
+        return n;
+    }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (int rc = flush_timers(c)) return rc;
     const int n = int(c->timers.size());
@@ -2115,6 +2284,7 @@ const char *lulesh_hip_last_error(const lulesh_ctx *c) { return c ? c->err.c_str
 
 void lulesh_hip_destroy(lulesh_ctx *c) {
     if (!c) return;
+    for (lulesh_ctx *l : c->lanes) lulesh_hip_destroy(l);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     for (void *p : c->allocs) (void)hipFree(p);

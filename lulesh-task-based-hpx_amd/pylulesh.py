@@ -15,6 +15,13 @@ LIB_PATH = os.path.join(HERE, "liblulesh_amd.so")
 OK, VOLUME_ERROR, QSTOP_ERROR, INVALID_ARGUMENT = 0, -1, -2, -3
 FP64, MIXED = 0, 1
 OPT_NODE_FORCES, OPT_GENERIC = 0x100, 0x200
+
+
+def OPT_STREAMS(k):
+    """Option splitting a batch over k streams (LULESH_OPT_STREAMS)."""
+    return (int(k) & 0xF) << 12
+
+
 PHASE_NODAL, PHASE_ELEMENTS, PHASE_MATERIAL, PHASE_CONSTRAINTS = 0, 1, 2, 3
 
 FIELDS = ("x y z xd yd zd xdd ydd zdd fx fy fz nodalMass e p q ql qq v volo delv vdov "

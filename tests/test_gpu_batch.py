@@ -127,3 +127,58 @@ def test_mixed_batch(pyl):
         e64 = dev.download("e")[0]
         e32 = batch.download(k, "e")[0]
         assert abs(e32 - e64) <= 1e-6 * abs(e64)
+
+
+@pytest.mark.parametrize("k", [2, 3])
+def test_stream_groups_equal_one_stream(pyl, k):
+    """A batch split over k streams (each driven by its own host thread)
+    leaves every problem exactly as the one-stream batch does."""
+    banner(f"batch over {k} streams vs one stream")
+    params = [(5, 11, 1.0), (9, 16, 0.6), (12, 21, 1.7), (7, 11, 1.3), (16, 16, 0.9)]
+    sizes = [p[0] for p in params]
+    regions = [p[1] for p in params]
+    factors = [p[2] for p in params]
+    its = 60
+    one = pyl.Batch.sedov(sizes, regions, factors)
+    rc1, s1 = one.run(its)
+    grp = pyl.Batch.sedov(sizes, regions, factors, opts=pyl.OPT_STREAMS(k))
+    rc2, s2 = grp.run(its)
+    assert rc1 == rc2 == 0
+    for p in range(len(params)):
+        assert s1[p].cycle == s2[p].cycle == its
+        assert s1[p].time == s2[p].time and s1[p].deltatime == s2[p].deltatime
+        for name in FIELDS:
+            np.testing.assert_array_equal(grp.download(p, name), one.download(p, name),
+                                          err_msg=f"problem {p} {name}")
+    # a second run continues every lane from where it stopped (the smallest
+    # problems reach the stop time on the way)
+    rc1, s1 = one.run(its + 25)
+    rc2, s2 = grp.run(its + 25)
+    assert rc1 == rc2 == 0
+    assert max(st.cycle for st in s1) == its + 25
+    for p in range(len(params)):
+        assert s1[p].cycle == s2[p].cycle and s1[p].time == s2[p].time
+        np.testing.assert_array_equal(grp.download(p, "e"), one.download(p, "e"))
+    buf = np.empty(8, np.float64)
+    assert grp.L.lulesh_hip_batch_download(grp.h, len(params), pyl.FIELD_ID["e"],
+                                           buf.ctypes.data, 8) == pyl.INVALID_ARGUMENT
+
+
+def test_stream_group_host_domains_and_tuning(pyl):
+    """The group built from host Domains, autotuned per lane and given an
+    explicit tuning, still equals the individual runs."""
+    banner("stream group from host Domains, autotune")
+    params = [(6, 11, 1.0), (10, 21, 1.8), (8, 16, 0.7)]
+    its = 40
+    grp = pyl.Batch([pyl.make_domain(*p) for p in params], opts=pyl.OPT_STREAMS(2))
+    t, ms = grp.autotune(4)
+    assert ms > 0 and t.zplanes >= 1
+    grp.set_tuning(pyl.Tuning(3, 64, 128, 4, 1))
+    rc, states = grp.run(its)
+    assert rc == 0
+    for p, prm in enumerate(params):
+        dev, ts = single_run(pyl, *prm, its)
+        assert states[p].cycle == ts.cycle and states[p].time == ts.time
+        for name in FIELDS:
+            np.testing.assert_array_equal(grp.download(p, name), dev.download(name),
+                                          err_msg=f"problem {p} {name}")
