@@ -31,6 +31,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -763,6 +764,7 @@ struct lulesh_ctx {
     bool structured = false;
     bool store_forces = false; // also write fx.. and xdd.. (Domain fidelity)
     int nprob = 0;
+    int cus = 256; // compute units of the device
     std::vector<ProbHost> probs;
     int64_t totE = 0, totN = 0;
     Params P{};
@@ -903,12 +905,38 @@ enum { T_TIME = 0, T_NODAL, T_NODES, T_ELEMS, T_MATERIAL, T_MINIMA, T_COUNT };
 // largest candidate that still gives the launch about eight workgroups per
 // CU (two resident per CU, four rounds); 4 for small problems.
 int auto_zplanes(const lulesh_ctx *c) {
+    if (c->nprob == 1) {
+        // One problem: the count that minimises (rounds of workgroups) x
+        // (element planes per workgroup), a round being the two z-march
+        // workgroups per CU that fit at once, among the counts that fill at
+        // least 90% of a round (all counts compete when none does).  This
+        // model's pick was the fastest of every group count measured at
+        // s = 150 and s = 250.
+        const int n = c->probs[0].s + 1;
+        const int64_t t2 = int64_t(zmarch_tiles(n)) * zmarch_tiles(n);
+        const int64_t slots = 2 * int64_t(std::max(c->cus, 1));
+        bool any_full = false;
+        for (int g = 1; g <= n && !any_full; ++g) any_full = 10 * t2 * g >= 9 * slots;
+        int best = 1;
+        int64_t best_cost = INT64_MAX;
+        for (int g = 1; g <= n; ++g) {
+            const int z = (n + g - 1) / g;
+            const int64_t wgs = t2 * ((n + z - 1) / z);
+            if (any_full && 10 * wgs < 9 * slots) continue;
+            const int64_t cost = ((wgs + slots - 1) / slots) * (z + 1);
+            if (cost <= best_cost) {
+                best_cost = cost;
+                best = z;
+            }
+        }
+        return best;
+    }
     int nmax = 1;
     for (const auto &p : c->probs) nmax = std::max(nmax, p.s + 1);
     auto blocks = [&](int z) {
         int64_t b = 0;
         for (const auto &p : c->probs) {
-            const int n = p.s + 1, t = (n + kTile - 1) / kTile;
+            const int n = p.s + 1, t = zmarch_tiles(n);
             b += int64_t(t) * t * ((n + z - 1) / z);
         }
         return b;
@@ -940,7 +968,7 @@ int configure(lulesh_ctx *c) {
         d.elemOff = ph.elemOff;
         d.nodeOff = ph.nodeOff;
         const int n = ph.s + 1;
-        d.tiles = (n + kTile - 1) / kTile;
+        d.tiles = zmarch_tiles(n);
         d.inv_s = 1.0 / double(ph.s);
         d.inv_ss = 1.0 / (double(ph.s) * double(ph.s));
         d.zplanes = std::min(t.zplanes, n);
@@ -1336,8 +1364,11 @@ int create_impl(const lulesh_domain_view *const *views, int nprob, int precision
     c->timers = {{"time_increment"}, {"nodal"}, {"nodes"}, {"elements"}, {"material"},
                  {"minima"}};
     int rc;
+    int dev = 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->run0) != hipSuccess || hipEventCreate(&c->run1) != hipSuccess)
+        hipEventCreate(&c->run0) != hipSuccess || hipEventCreate(&c->run1) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return fail(LULESH_HIP_ERROR);
     {
         int32_t eo = 0, no = 0;
@@ -1701,8 +1732,11 @@ int create_sedov_impl(const int32_t *sizes, const int32_t *regions, const double
     c->hgcoef = D::m_hgcoef;
     c->timers = {{"time_increment"}, {"nodal"}, {"nodes"}, {"elements"}, {"material"},
                  {"minima"}};
+    int dev = 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->run0) != hipSuccess || hipEventCreate(&c->run1) != hipSuccess)
+        hipEventCreate(&c->run0) != hipSuccess || hipEventCreate(&c->run1) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return fail(LULESH_HIP_ERROR);
     {
         int32_t eo = 0, no = 0;
@@ -1845,6 +1879,7 @@ std::vector<std::vector<int32_t>> deal_problems(const std::vector<int64_t> &E, i
 template <typename Make>
 int make_group(const std::vector<std::vector<int32_t>> &parts, int nprob, Make &&make,
                lulesh_ctx **out) {
+    *out = nullptr;
     auto *g = new lulesh_ctx();
     g->nprob = nprob;
     g->lane_of.assign(size_t(nprob), 0);
@@ -2185,14 +2220,23 @@ int lulesh_hip_autotune(lulesh_ctx *c, int32_t cycles, lulesh_tuning *best,
     double cur_ms = 1e300;
     if (rc == 0) rc = trial(cur, &cur_ms);
     if (rc == 0) cur = c->tune;
-    // coordinate descent over the launch parameters
-    const int zcand[] = {2, 3, 4, 5, 6, 8, 10, 12, 16, 24, 32};
+    // coordinate descent over the launch parameters; the plane counts are
+    // the smallest ones giving 2..40 workgroups along z for the largest
+    // problem (each of them splits its node planes as evenly as a count can)
+    int nmax = 1;
+    for (const auto &p : c->probs) nmax = std::max(nmax, p.s + 1);
+    std::vector<int> zcand;
+    for (int g = 40; g >= 2; --g) {
+        const int z = (nmax + g - 1) / g;
+        if (z >= 2 && (zcand.empty() || zcand.back() != z)) zcand.push_back(z);
+    }
     const int bcand[] = {64, 128, 256};
     const int gcand[] = {4, 8, 16, 32};
     const int mcand[] = {0, 1};
     for (int pass = 0; pass < 5 && rc == 0; ++pass) {
-        const int *cand = pass == 0 ? zcand : pass == 3 ? gcand : pass == 4 ? mcand : bcand;
-        const int ncand = pass == 0 ? 11 : pass == 3 ? 4 : pass == 4 ? 2 : 3;
+        const int *cand =
+            pass == 0 ? zcand.data() : pass == 3 ? gcand : pass == 4 ? mcand : bcand;
+        const int ncand = pass == 0 ? int(zcand.size()) : pass == 3 ? 4 : pass == 4 ? 2 : 3;
         for (int k = 0; k < ncand && rc == 0; ++k) {
             lulesh_tuning t = cur;
             int32_t *field = pass == 0   ? &t.zplanes

@@ -8,11 +8,14 @@
 //
 // k_nodal_zmarch fuses the whole nodal phase (lulesh.cc:2017-2110) into one
 // pass without storing corner forces in HBM.  A workgroup owns a tile of
-// 15x15 node columns over a range of node planes and marches upward in z:
-// for each element plane k its 16x16 threads compute the stress and
-// hourglass corner forces of the 16x16 element patch covering the tile
-// (one element of halo on the low x/y side), stage them in LDS, and the
-// owning threads add them to their nodes in the reference's corner order
+// node columns over a range of node planes and marches upward in z: for
+// each element plane k its 16x16 threads compute the stress and hourglass
+// corner forces of the 16x16 element patch covering the tile, stage them in
+// LDS, and the owning threads add them to their nodes.  Along x (and y) the
+// first tile owns node columns 0..15 of patch element columns 0..15; tile
+// t >= 1 owns the 15 node columns 15t+1 .. 15t+15 and its patch starts one
+// element lower (15t), the halo its first node column needs.  The node
+// forces are summed in the reference's corner order
 // (ascending element index, stress then hourglass per corner,
 // lulesh.cc:849-857):
 //   node plane k+1 starts with corners 6,7,5,4 of element plane k,
@@ -28,9 +31,14 @@
 namespace lulesh_dev {
 
 constexpr int kPatch = 16;            // element patch edge (threads per tile edge)
-constexpr int kTile = kPatch - 1;     // node tile edge owned by a workgroup
+constexpr int kTile = kPatch - 1;     // node columns owned by a tile after the first
 constexpr int kZmarchThreads = kPatch * kPatch;
 constexpr int kPatchNodes = (kPatch + 1) * (kPatch + 1); // node columns of a patch
+
+// Tiles along an edge of n nodes: 16 node columns, then 15 per tile.
+__host__ __device__ inline int zmarch_tiles(int n) {
+    return n <= kPatch ? 1 : 1 + (n - kPatch + kTile - 1) / kTile;
+}
 
 struct NodeIn {
     const double *x, *y, *z, *xd, *yd, *zd;
@@ -47,7 +55,7 @@ struct NodeForceOut { // optional copies of fx.. / xdd.. (Domain fidelity)
 struct ProbDesc {
     int32_t s, E, N;
     int32_t elemOff, nodeOff; // offsets into the concatenated element / node arrays
-    int32_t zplanes, tiles;   // z-march: node planes per workgroup, node tiles per edge
+    int32_t zplanes, tiles;   // z-march: node planes per workgroup, tiles per edge
     int32_t zBlock0, eBlock0; // first workgroup of this problem in the z-march / element grids
     int32_t mBlock0, pad;     // first workgroup in the element-order material grid
     // 1/s and 1/s^2 rounded to double: element (k, j, i) from its index li
@@ -102,22 +110,26 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
     const int jj = tid / kPatch, ii = tid % kPatch;
     const int tilesX = pd.tiles;
     const int tile = lb % (tilesX * tilesX);
-    const int J0 = (tile / tilesX) * kTile, I0 = (tile % tilesX) * kTile;
+    const int ty = tile / tilesX, tx = tile % tilesX;
+    const int dy = ty > 0, dx = tx > 0;           // 1: the patch has a low-side halo
+    const int E0y = ty * kTile, E0x = tx * kTile; // first element row / column of the patch
     const int K0 = (lb / (tilesX * tilesX)) * planes_per_block;
     const int K1 = min(K0 + planes_per_block, n); // owned node planes [K0, K1)
-    const int ej = J0 - 1 + jj, ei = I0 - 1 + ii; // this thread's element column
-    const bool elemCol = (ej >= 0) && (ej < s) && (ei >= 0) && (ei < s);
-    const int J = J0 + jj, I = I0 + ii; // this thread's node column
-    const bool ownsNode = (jj < kTile) && (ii < kTile) && (J < n) && (I < n);
+    const int ej = E0y + jj, ei = E0x + ii;        // this thread's element column
+    const bool elemCol = (ej < s) && (ei < s);
+    const int J = E0y + dy + jj, I = E0x + dx + ii; // this thread's node column
+    const bool ownsNode = (jj < kPatch - dy) && (ii < kPatch - dx) && (J < n) && (I < n);
     // validity of the four elements around node (J,I) in a plane:
-    // (J-1,I-1) (J-1,I) (J,I-1) (J,I)  <->  patch (jj,ii) (jj,ii+1) (jj+1,ii) (jj+1,ii+1)
+    // (J-1,I-1) (J-1,I) (J,I-1) (J,I)  <->  patch threads tb, tb+1, tb+16, tb+17
+    // (tb < 0 only where J = 0 or I = 0, whose elements are not read)
     const bool v00 = (J >= 1) && (I >= 1), v01 = (J >= 1) && (I < s), v10 = (J < s) && (I >= 1),
                v11 = (J < s) && (I < s);
+    const int tb = tid + (dy - 1) * kPatch + (dx - 1);
     const double dt = st->deltatime;
 
     double px = 0.0, py = 0.0, pz = 0.0; // partial sums of the node in the current plane
 
-    const int own = (jj + 1) * (kPatch + 1) + (ii + 1); // node (J, I) in the patch node array
+    const int own = (jj + dy) * (kPatch + 1) + (ii + dx); // node (J, I) in the patch node array
     auto finish_node = [&](int K, double mass) {
         const int g = (K * n + J) * n + I;
         const int slot = K & 1;
@@ -160,8 +172,8 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
     for (int m = 0; m < 2; ++m) {
         const int t = tid + m * kZmarchThreads;
         const int a = t / (kPatch + 1), b = t - a * (kPatch + 1);
-        const int Jn = J0 - 1 + a, In = I0 - 1 + b;
-        pin[m] = t < kPatchNodes && Jn >= 0 && Jn < n && In >= 0 && In < n;
+        const int Jn = E0y + a, In = E0x + b;
+        pin[m] = t < kPatchNodes && Jn < n && In < n;
         pg[m] = Jn * n + In;
     }
     double rn[2][6];
@@ -335,10 +347,10 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
         __syncthreads();
         // ---- node plane k ends with corners 2,3,1,0 ----------------------------
         if (ownsNode && k >= K0) {
-            if (v00) add(2, tid);
-            if (v01) add(3, tid + 1);
-            if (v10) add(1, tid + kPatch);
-            if (v11) add(0, tid + kPatch + 1);
+            if (v00) add(2, tb);
+            if (v01) add(3, tb + 1);
+            if (v10) add(1, tb + kPatch);
+            if (v11) add(0, tb + kPatch + 1);
             finish_node(k, mcur);
         }
         __syncthreads();
@@ -351,10 +363,10 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
         // lds now holds corners 4..7 at slots 0..3
         px = py = pz = 0.0;
         if (ownsNode && k + 1 < K1) {
-            if (v00) add(2, tid);             // corner 6 of element (J-1, I-1)
-            if (v01) add(3, tid + 1);         // corner 7 of element (J-1, I)
-            if (v10) add(1, tid + kPatch);    // corner 5 of element (J, I-1)
-            if (v11) add(0, tid + kPatch + 1); // corner 4 of element (J, I)
+            if (v00) add(2, tb);              // corner 6 of element (J-1, I-1)
+            if (v01) add(3, tb + 1);          // corner 7 of element (J-1, I)
+            if (v10) add(1, tb + kPatch);     // corner 5 of element (J, I-1)
+            if (v11) add(0, tb + kPatch + 1); // corner 4 of element (J, I)
         }
         mcur = mnext;
     }
