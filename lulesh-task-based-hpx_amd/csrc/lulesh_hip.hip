@@ -373,19 +373,21 @@ __global__ __launch_bounds__(kBlock) void k_elems(
         load_nodelist(nodelist, li, n);
     }
     double x[8], y[8], z[8], xd[8], yd[8], zd[8];
+    // every load is issued before the first store (the stores could alias
+    // the node arrays as far as the compiler knows)
     load8(nf[0], n, x);
     load8(nf[1], n, y);
     load8(nf[2], n, z);
-    const double volume = elem_volume<double>(x, y, z);
-    const double voloi = f0.volo[i];
-    const double rel = volume / voloi;
-    const double vold = f0.v[i];
-    f0.vnew[i] = rel;
-    f0.delv[i] = rel - vold;
-    f0.arealg[i] = characteristic_length(x, y, z, volume);
     load8(nf[3], n, xd);
     load8(nf[4], n, yd);
     load8(nf[5], n, zd);
+    const double voloi = f0.volo[i];
+    const double vold = f0.v[i];
+    const double volume = elem_volume<double>(x, y, z);
+    const double rel = volume / voloi;
+    f0.vnew[i] = rel;
+    f0.delv[i] = rel - vold;
+    f0.arealg[i] = characteristic_length(x, y, z, volume);
     // Q gradients on the end-of-step geometry (lulesh.cc:1349)
     {
         QG g[6];
