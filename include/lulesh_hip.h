@@ -193,7 +193,10 @@ int lulesh_hip_get_tuning(const lulesh_ctx *ctx, lulesh_tuning *tuning);
  * (coordinate descent over zplanes, block_elem, block_material,
  * cycles_per_graph, material_mode; each candidate is run twice from the
  * snapshot, the second run timed) starting from a snapshot of the current state, restores
- * the state, and keeps the fastest.  Returns it and its ms per cycle. */
+ * the state, and keeps the fastest.  Returns it and its ms per cycle.  A
+ * stream group (LULESH_OPT_STREAMS) is tuned as a whole: every candidate is
+ * set on all streams, which run at once, and the slowest stream's device
+ * time counts. */
 int lulesh_hip_autotune(lulesh_ctx *ctx, int32_t cycles, lulesh_tuning *best,
                         double *best_ms_per_cycle);
 

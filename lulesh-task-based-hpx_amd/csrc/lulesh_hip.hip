@@ -2153,80 +2153,114 @@ int lulesh_hip_get_tuning(const lulesh_ctx *c, lulesh_tuning *t) {
 int lulesh_hip_autotune(lulesh_ctx *c, int32_t cycles, lulesh_tuning *best,
                         double *best_ms_per_cycle) {
     if (!c || cycles < 2) return LULESH_INVALID_ARGUMENT;
-    if (!c->lanes.empty()) {
-        // each lane on its own (the lanes are timed one after another)
-        double sum = 0;
-        for (lulesh_ctx *l : c->lanes) {
-            double ms = 0;
-            if (int rc = lulesh_hip_autotune(l, cycles, nullptr, &ms)) return rc;
-            sum += ms;
+    // The contexts timed together: the context itself, or every lane of a
+    // stream group (one tuning for all lanes, the lanes running at once as
+    // lulesh_hip_batch_run runs them).
+    std::vector<lulesh_ctx *> cs;
+    if (c->lanes.empty())
+        cs.push_back(c);
+    else
+        cs = c->lanes;
+    const size_t L = cs.size();
+    // Snapshot every state buffer of every context; each candidate runs
+    // `cycles` cycles from the snapshot and the state is restored afterwards.
+    std::vector<Snapshot> snaps(L);
+    std::vector<std::vector<DevState>> st0(L);
+    std::vector<char> timing0(L);
+    auto free_snaps = [&]() {
+        for (auto &sn : snaps)
+            for (void *q : sn.copies) (void)hipFree(q);
+    };
+    int rc = 0;
+    for (size_t l = 0; l < L && rc == 0; ++l) {
+        lulesh_ctx *x = cs[l];
+        Snapshot &snap = snaps[l];
+        for (int id = 0; id < LULESH_F_COUNT; ++id)
+            if (x->f[id] && !is_state_node_field(id))
+                snap.regions.push_back(
+                    {x->f[id], size_t(is_node_field(id) ? x->totN : x->totE) * 8});
+        for (int k = 0; k < 6; ++k) {
+            if (x->g32[k]) snap.regions.push_back({x->g32[k], size_t(x->totE) * 4});
+            snap.regions.push_back({x->nb.b[0][k], size_t(x->totN) * 8});
+            if (x->nb.b[1][k] != x->nb.b[0][k])
+                snap.regions.push_back({x->nb.b[1][k], size_t(x->totN) * 8});
         }
-        c->tune = c->lanes[0]->tune;
-        if (best) *best = c->tune;
-        if (best_ms_per_cycle) *best_ms_per_cycle = sum;
+        snap.regions.push_back({x->st, sizeof(DevState) * size_t(x->nprob)});
+        for (auto &r : snap.regions) {
+            void *q = nullptr;
+            if (hipMalloc(&q, r.second) != hipSuccess) {
+                free_snaps();
+                return hip_fail(x, hipErrorOutOfMemory, "autotune snapshot");
+            }
+            snap.copies.push_back(q);
+            (void)hipMemcpyAsync(q, r.first, r.second, hipMemcpyDeviceToDevice, x->stream);
+        }
+        timing0[l] = x->timing;
+        x->timing = false;
+        rc = read_states(x);
+        st0[l].assign(x->hst, x->hst + x->nprob);
+    }
+    if (rc) {
+        for (size_t l = 0; l < L; ++l) cs[l]->timing = timing0[l] != 0;
+        free_snaps();
+        return rc;
+    }
+    auto restore = [&](size_t l) -> int {
+        lulesh_ctx *x = cs[l];
+        for (size_t k = 0; k < snaps[l].regions.size(); ++k)
+            HIP_TRY(x, hipMemcpyAsync(snaps[l].regions[k].first, snaps[l].copies[k],
+                                      snaps[l].regions[k].second, hipMemcpyDeviceToDevice,
+                                      x->stream));
+        return read_states(x);
+    };
+    // restore the snapshot and arm `cycles` cycles
+    auto rearm = [&](size_t l) -> int {
+        lulesh_ctx *x = cs[l];
+        int r = restore(l);
+        if (r) return r;
+        for (int pi = 0; pi < x->nprob; ++pi) x->hst[pi].its = st0[l][size_t(pi)].cycle + cycles;
+        return write_states(x);
+    };
+    // run_loop on every context at once (lane 0 on this thread)
+    auto run_all = [&]() -> int {
+        std::vector<int> rcs(L, 0);
+        std::vector<std::thread> th;
+        for (size_t l = 1; l < L; ++l) th.emplace_back([&, l] { rcs[l] = run_loop(cs[l]); });
+        rcs[0] = run_loop(cs[0]);
+        for (auto &t : th) t.join();
+        for (int r : rcs)
+            if (r) return r;
         return 0;
-    }
-    // Snapshot every state buffer; each candidate runs `cycles` cycles from
-    // the snapshot and the state is restored afterwards.
-    Snapshot snap;
-    for (int id = 0; id < LULESH_F_COUNT; ++id)
-        if (c->f[id] && !is_state_node_field(id))
-            snap.regions.push_back({c->f[id], size_t(is_node_field(id) ? c->totN : c->totE) * 8});
-    for (int k = 0; k < 6; ++k) {
-        if (c->g32[k]) snap.regions.push_back({c->g32[k], size_t(c->totE) * 4});
-        snap.regions.push_back({c->nb.b[0][k], size_t(c->totN) * 8});
-        if (c->nb.b[1][k] != c->nb.b[0][k])
-            snap.regions.push_back({c->nb.b[1][k], size_t(c->totN) * 8});
-    }
-    snap.regions.push_back({c->st, sizeof(DevState) * size_t(c->nprob)});
-    for (auto &r : snap.regions) {
-        void *q = nullptr;
-        if (hipMalloc(&q, r.second) != hipSuccess) {
-            for (void *p : snap.copies) (void)hipFree(p);
-            return hip_fail(c, hipErrorOutOfMemory, "autotune snapshot");
-        }
-        (void)hipMemcpyAsync(q, r.first, r.second, hipMemcpyDeviceToDevice, c->stream);
-        snap.copies.push_back(q);
-    }
-    const bool timing0 = c->timing;
-    c->timing = false;
-    int rc = read_states(c);
-    std::vector<DevState> st0(c->hst, c->hst + c->nprob);
-    auto restore = [&]() -> int {
-        for (size_t k = 0; k < snap.regions.size(); ++k)
-            HIP_TRY(c, hipMemcpyAsync(snap.regions[k].first, snap.copies[k],
-                                      snap.regions[k].second, hipMemcpyDeviceToDevice,
-                                      c->stream));
-        return read_states(c);
     };
     auto trial = [&](const lulesh_tuning &t, double *ms) -> int {
-        c->tune = t;
-        int r = configure(c);
-        if (r) return r;
-        if ((r = restore())) return r;
-        for (int pi = 0; pi < c->nprob; ++pi) c->hst[pi].its = st0[size_t(pi)].cycle + cycles;
-        if ((r = write_states(c))) return r;
-        if ((r = build_graph(c))) return r; // graph build outside the timing
-        // a first run warms the new graph and launch tables up; the second,
-        // from the same snapshot, is the one timed
-        if ((r = run_loop(c))) return r;
-        if ((r = restore())) return r;
-        for (int pi = 0; pi < c->nprob; ++pi) c->hst[pi].its = st0[size_t(pi)].cycle + cycles;
-        if ((r = write_states(c))) return r;
-        r = run_loop(c);
-        *ms = c->last_run_ms / cycles;
-        return r;
+        int r = 0;
+        for (size_t l = 0; l < L && r == 0; ++l) {
+            cs[l]->tune = t;
+            if (!(r = configure(cs[l])) && !(r = rearm(l)))
+                r = build_graph(cs[l]); // graph build outside the timing
+        }
+        // a first run warms the new graphs and launch tables up; the second,
+        // from the same snapshot, is the one timed (device time of the
+        // slowest context)
+        if (r || (r = run_all())) return r;
+        for (size_t l = 0; l < L && r == 0; ++l) r = rearm(l);
+        if (r || (r = run_all())) return r;
+        double slowest = 0;
+        for (lulesh_ctx *x : cs) slowest = std::max(slowest, x->last_run_ms);
+        *ms = slowest / cycles;
+        return 0;
     };
-    lulesh_tuning cur = c->tune;
-    cur.zplanes = auto_zplanes(c);
+    lulesh_tuning cur = cs[0]->tune;
+    cur.zplanes = auto_zplanes(cs[0]);
     double cur_ms = 1e300;
-    if (rc == 0) rc = trial(cur, &cur_ms);
-    if (rc == 0) cur = c->tune;
+    rc = trial(cur, &cur_ms);
+    if (rc == 0) cur = cs[0]->tune;
     // coordinate descent over the launch parameters; the plane counts are
     // the smallest ones giving 2..40 workgroups along z for the largest
     // problem (each of them splits its node planes as evenly as a count can)
     int nmax = 1;
-    for (const auto &p : c->probs) nmax = std::max(nmax, p.s + 1);
+    for (lulesh_ctx *x : cs)
+        for (const auto &p : x->probs) nmax = std::max(nmax, p.s + 1);
     std::vector<int> zcand;
     for (int g = 40; g >= 2; --g) {
         const int z = (nmax + g - 1) / g;
@@ -2251,16 +2285,20 @@ int lulesh_hip_autotune(lulesh_ctx *c, int32_t cycles, lulesh_tuning *best,
             double ms = 0;
             rc = trial(t, &ms);
             if (rc == 0 && ms < cur_ms) {
-                cur = c->tune;
+                cur = cs[0]->tune;
                 cur_ms = ms;
             }
         }
     }
+    int rc2 = 0, rc3 = 0;
+    for (size_t l = 0; l < L; ++l) {
+        cs[l]->tune = cur;
+        if (int r = configure(cs[l])) rc2 = rc2 ? rc2 : r;
+        if (int r = restore(l)) rc3 = rc3 ? rc3 : r;
+        cs[l]->timing = timing0[l] != 0;
+    }
+    free_snaps();
     c->tune = cur;
-    const int rc2 = configure(c);
-    const int rc3 = restore();
-    for (void *p : snap.copies) (void)hipFree(p);
-    c->timing = timing0;
     if (rc) return rc;
     if (rc2) return rc2;
     if (rc3) return rc3;
