@@ -64,16 +64,19 @@ def kernel_bytes(name, E, N, mixed=False, material_mode=0):
     return 0
 
 
-TRAFFIC_FILE = "profiles/hbm_traffic_sedov_s150_fp64.md"
+# committed rocprofv3 summaries of the s=150 FP64 leg, per arithmetic form
+TRAFFIC_FILES = {"fma": "profiles/hbm_traffic_sedov_s150_fp64.md",
+                 "exact": "profiles/hbm_traffic_sedov_s150_fp64_exact.md"}
 PROF_CYCLES = 64  # cycles of the per-kernel event timing pass
 
 
-def measured(key, kernel):
+def measured(key, kernel, arith="fma"):
     """A per-launch figure of `kernel` at s=150 FP64 from the committed
-    rocprofv3 summary: key "traffic" = HBM bytes (FETCH_SIZE + WRITE_SIZE,
-    copy-calibrated), "valu_busy" = fraction of time the VALUs issue,
-    "fp64_flop" = FP64 lane operations (FMA = 2).  None if absent."""
-    path = os.path.join(ROOT, TRAFFIC_FILE)
+    rocprofv3 summary of that arithmetic form: key "traffic" = HBM bytes
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated), "valu_busy" = fraction of
+    time the VALUs issue, "fp64_flop" = FP64 lane operations (FMA = 2).
+    None if absent."""
+    path = os.path.join(ROOT, TRAFFIC_FILES[arith])
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -119,7 +122,7 @@ def timed_run(pyl, dev, dom, warmup, steps, tune_cycles):
     return t1 - t0, ts, tuning
 
 
-def batch_bench(pyl, nprob, its, seed, tune_cycles, precision, streams=1):
+def batch_bench(pyl, nprob, its, seed, tune_cycles, precision, streams=1, arith_opts=0):
     """The ensemble: nprob Sedov problems with seeded sizes in [20, 80],
     region counts in {11, 16, 21} and energy factors in [0.5, 2), each run
     for `its` cycles (or to its stop time) in one context, split over
@@ -127,7 +130,7 @@ def batch_bench(pyl, nprob, its, seed, tune_cycles, precision, streams=1):
     sizes, regions, factors = pyl.batch_params(seed, nprob)
     # built in HBM: bitwise the batch of host Domains (tests/test_gpu_setup.py)
     b = pyl.Batch.sedov(sizes, regions, factors, precision=precision,
-                        opts=pyl.OPT_STREAMS(streams) if streams > 1 else 0)
+                        opts=arith_opts | (pyl.OPT_STREAMS(streams) if streams > 1 else 0))
     tuning = None
     if tune_cycles:
         t, _ = b.autotune(tune_cycles)
@@ -176,6 +179,13 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the s=250 line")
+    ap.add_argument("--arith", choices=("fma", "exact"), default="fma",
+                    help="arithmetic form of every line: multiply-adds fused (as the "
+                         "reference's GCC -O3 -mfma build) or every operation rounded on its "
+                         "own (bitwise the oracle); the s=150 FP64 line of the other form "
+                         "is reported beside the headline")
+    ap.add_argument("--no-other-form", action="store_true",
+                    help="skip the s=150 FP64 line of the other arithmetic form")
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--batch-its", type=int, default=1000)
     ap.add_argument("--batch-streams", type=int, default=2,
@@ -204,7 +214,8 @@ def main():
     print(f"[bench] building Domain s={s}", file=sys.stderr, flush=True)
     dom = pyl.Domain(s, nr)
     t_up = time.perf_counter()
-    dev = pyl.Device(dom, pyl.FP64)  # allocation, upload of every field, layout checks
+    AO = pyl.OPT_FMA if args.arith == "fma" else 0
+    dev = pyl.Device(dom, pyl.FP64, AO)  # allocation, upload of every field, layout checks
     t_up = time.perf_counter() - t_up
     elapsed, ts, tuning = timed_run(pyl, dev, dom, args.warmup, args.steps, args.tune_cycles)
     dev_ms = dev.last_run_ms()  # HIP events around the replayed graphs of the timed region
@@ -243,9 +254,9 @@ def main():
     dk = kernels[dom_k]
     achieved = dk["bytes_per_launch"] / (dk["avg_ms"] * 1e-3) / 1e9
     prof = (s, nr) == (150, 11)
-    traffic = measured("traffic", dom_k) if prof else None
-    valu_busy = measured("valu_busy", dom_k) if prof else None
-    fp64_flop = measured("fp64_flop", dom_k) if prof else None
+    traffic = measured("traffic", dom_k, args.arith) if prof else None
+    valu_busy = measured("valu_busy", dom_k, args.arith) if prof else None
+    fp64_flop = measured("fp64_flop", dom_k, args.arith) if prof else None
     fp64 = None
     if fp64_flop:
         tf = fp64_flop / (dk["avg_ms"] * 1e-3) / 1e12
@@ -255,19 +266,54 @@ def main():
     ms_cycle_events = sum(k["avg_ms"] * k["launches_per_cycle"] for k in kernels.values())
     dev.close()
 
+    # ---- the same FP64 line in the other arithmetic form ------------------
+    other = None
+    e0_exact = e0_fp64 if args.arith == "exact" else None
+    if not args.no_other_form:
+        oname = "exact" if args.arith == "fma" else "fma"
+        domo = pyl.Domain(s, nr)
+        devo = pyl.Device(domo, pyl.FP64, pyl.OPT_FMA if oname == "fma" else 0)
+        el_o, tso, _ = timed_run(pyl, devo, domo, args.warmup, args.steps, args.tune_cycles)
+        dev_ms_o = devo.last_run_ms()
+        devo.set_timing(True)
+        rc = devo.run(tso.cycle + PROF_CYCLES, tso)  # to the cycle the headline state is read at
+        assert rc == 0, rc
+        kto = devo.kernel_times()
+        devo.set_timing(False)
+        devo.download_view()
+        e0o = float(domo.field("e")[0])
+        repo = domo.report()
+        if oname == "exact":
+            e0_exact = e0o
+        other = {"arith": oname, "value": E * args.steps / el_o,
+                 "ms_per_step": 1e3 * el_o / args.steps,
+                 "device_ms_per_step": dev_ms_o / args.steps,
+                 "kernels_avg_ms": {k: round(a, 5) for k, (a, n) in kto.items() if n},
+                 "origin_energy": e0o,
+                 "origin_energy_rel_diff_vs_headline": abs(e0o - e0_fp64) / abs(e0_fp64),
+                 "symmetry_max_rel_diff": float(repo[3]), "cycles": tso.cycle}
+        devo.close()
+        domo.close()
+        print(f"[bench] fp64 {oname}: {other['value']:.4e} zc/s", file=sys.stderr, flush=True)
+
     # ---- FP32-mixed form -------------------------------------------------
     mixed = None
     if not args.no_mixed:
         domm = pyl.Domain(s, nr)
-        devm = pyl.Device(domm, pyl.MIXED)
+        devm = pyl.Device(domm, pyl.MIXED, AO)
         el_m, tsm, _ = timed_run(pyl, devm, domm, args.warmup, args.steps, args.tune_cycles)
         rc = devm.run(tsm.cycle + PROF_CYCLES, tsm)  # to the cycle the FP64 state is read at
         assert rc == 0, rc
         devm.download_view()
         e0m = float(domm.field("e")[0])
         repm = domm.report()
+        e0ref = e0_exact if e0_exact is not None else e0_fp64
         mixed = {"value": E * args.steps / el_m, "ms_per_step": 1e3 * el_m / args.steps,
-                 "origin_energy_rel_err_vs_fp64": abs(e0m - e0_fp64) / abs(e0_fp64),
+                 "arith": args.arith,
+                 "origin_energy_rel_err_vs_fp64": abs(e0m - e0ref) / abs(e0ref),
+                 "origin_energy_reference": "FP64 every operation rounded on its own "
+                                            "(bitwise the oracle), same cycle"
+                                            if e0_exact is not None else "FP64 headline form",
                  "symmetry_max_abs_diff": float(repm[1]), "symmetry_max_rel_diff": float(repm[3]),
                  "cycles": tsm.cycle}
         devm.close()
@@ -280,7 +326,7 @@ def main():
                            "built in HBM (bitwise the host set-up)"}
         e0 = {}
         for name, prec in (("fp64", pyl.FP64), ("mixed", pyl.MIXED)):
-            dl = pyl.Device.sedov(250, 21, precision=prec)
+            dl = pyl.Device.sedov(250, 21, precision=prec, opts=AO)
             tl = dl.time_state()
             rc = dl.run(10, tl)
             assert rc == 0, rc
@@ -303,7 +349,7 @@ def main():
     if not args.no_batch:
         print(f"[bench] batch of {args.batch}", file=sys.stderr, flush=True)
         batch = batch_bench(pyl, args.batch, args.batch_its, 0, args.batch_tune_cycles, pyl.FP64,
-                            args.batch_streams)
+                            args.batch_streams, AO)
         print(f"[bench] batch: {batch['value']:.4e} zc/s", file=sys.stderr, flush=True)
 
     # ---- CPU baseline ----------------------------------------------------
@@ -338,14 +384,19 @@ def main():
         "data": "synthetic (deterministic Sedov initial conditions)",
         "config": {"workload": f"LULESH Sedov s={s} ({E} elements, {N} nodes), {nr} regions, "
                                "-b 1 -c 1, FP64, cycles from the start of the run",
-                   "s": s, "regions": nr, "elements": E},
+                   "s": s, "regions": nr, "elements": E,
+                   "arith": ("FP64 with multiply-adds fused (as the reference's GCC -O3 -mfma "
+                             "build)" if args.arith == "fma" else
+                             "FP64, every multiply and add rounded on its own (bitwise the "
+                             "oracle)")},
         "roofline": {"bound": "hbm", "kernel": dom_k, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": int(traffic) if traffic else None,
                      "valu_busy": valu_busy,
                      "fp64_vector": fp64,
-                     "source_of_traffic_and_valu_busy": TRAFFIC_FILE if traffic else None},
+                     "source_of_traffic_and_valu_busy": TRAFFIC_FILES[args.arith]
+                     if traffic else None},
         "model_bytes_per_zone_cycle": round(bsz, 1),
         "model_hbm_frac": round(bsz * zcps / (HBM_PEAK_GBS * 1e9), 4),
         "kernels": kernels,
@@ -353,6 +404,7 @@ def main():
         "value_incl_host_transfers": E * cycles / (elapsed + t_up + t_down),
         "host_transfer_s": {"create_and_upload": round(t_up, 4), "download": round(t_down, 4)},
         "origin_energy": e0_fp64,
+        "fp64_other_form": other,
         "symmetry_max_abs_diff": float(rep64[1]),
         "symmetry_max_rel_diff": float(rep64[3]),
         "tuning": tuning,

@@ -49,6 +49,12 @@ extern "C" {
                                         (they are dead between cycles; off by default) */
 #define LULESH_OPT_GENERIC 0x200     /* use the indexed kernels (nodelist + corner CSR)
                                         even when the mesh is the structured Sedov cube */
+#define LULESH_OPT_FMA 0x400        /* contract FP64 (and FP32) multiply-adds into fused
+                                        operations, as the reference's own GCC -O3 -mfma
+                                        build does: one rounding per multiply-add instead
+                                        of two.  Results then agree with the separately
+                                        rounded form to within a few ulps per operation
+                                        (not bit for bit). */
 /* Batched ensembles only: split the problems over k streams (2 <= k <= 15,
  * at most one per problem), each a complete context over its share (largest
  * problems first, each to the stream with the fewest elements so far) driven

@@ -97,32 +97,44 @@ __device__ __forceinline__ float sqrt_area(float x) { return sqrt_(x); }
 // asm statement (no instruction is emitted).
 __device__ __forceinline__ void opaque(double &v) { asm volatile("" : "+v"(v)); }
 
-// CalcElemVolume, lulesh.cc:592-674
+// CalcElemVolume, lulesh.cc:592-674.  The expression is written once and
+// used twice: elem_volume follows the translation unit's contraction setting
+// (the cycle's arithmetic form), elem_volume_rounded rounds every multiply
+// and add on its own in any form, so that the set-up in HBM gives the same
+// initial volumes and masses as the host set-up in both forms.
+#define LULESH_ELEM_VOLUME_BODY(T) \
+    const T twelveth = T(1.0) / T(12.0); \
+    const T dx61 = x[6] - x[1], dy61 = y[6] - y[1], dz61 = z[6] - z[1]; \
+    const T dx70 = x[7] - x[0], dy70 = y[7] - y[0], dz70 = z[7] - z[0]; \
+    const T dx63 = x[6] - x[3], dy63 = y[6] - y[3], dz63 = z[6] - z[3]; \
+    const T dx20 = x[2] - x[0], dy20 = y[2] - y[0], dz20 = z[2] - z[0]; \
+    const T dx50 = x[5] - x[0], dy50 = y[5] - y[0], dz50 = z[5] - z[0]; \
+    const T dx64 = x[6] - x[4], dy64 = y[6] - y[4], dz64 = z[6] - z[4]; \
+    const T dx31 = x[3] - x[1], dy31 = y[3] - y[1], dz31 = z[3] - z[1]; \
+    const T dx72 = x[7] - x[2], dy72 = y[7] - y[2], dz72 = z[7] - z[2]; \
+    const T dx43 = x[4] - x[3], dy43 = y[4] - y[3], dz43 = z[4] - z[3]; \
+    const T dx57 = x[5] - x[7], dy57 = y[5] - y[7], dz57 = z[5] - z[7]; \
+    const T dx14 = x[1] - x[4], dy14 = y[1] - y[4], dz14 = z[1] - z[4]; \
+    const T dx25 = x[2] - x[5], dy25 = y[2] - y[5], dz25 = z[2] - z[5]; \
+    /* grouping of the reference's TRIPLE_PRODUCT macro, lulesh.cc:649-651 */ \
+    auto triple = [](T x1, T y1, T z1, T x2, T y2, T z2, T x3, T y3, T z3) { \
+        return x1 * (y2 * z3 - z2 * y3) + x2 * (z1 * y3 - y1 * z3) + x3 * (y1 * z2 - z1 * y2); \
+    }; \
+    T volume = triple(dx31 + dx72, dx63, dx20, dy31 + dy72, dy63, dy20, dz31 + dz72, dz63, dz20) + \
+               triple(dx43 + dx57, dx64, dx70, dy43 + dy57, dy64, dy70, dz43 + dz57, dz64, dz70) + \
+               triple(dx14 + dx25, dx61, dx50, dy14 + dy25, dy61, dy50, dz14 + dz25, dz61, dz50); \
+    volume *= twelveth; \
+    return volume;
 template <typename T>
 __device__ __forceinline__ T elem_volume(const T x[8], const T y[8], const T z[8]) {
-    const T twelveth = T(1.0) / T(12.0);
-    const T dx61 = x[6] - x[1], dy61 = y[6] - y[1], dz61 = z[6] - z[1];
-    const T dx70 = x[7] - x[0], dy70 = y[7] - y[0], dz70 = z[7] - z[0];
-    const T dx63 = x[6] - x[3], dy63 = y[6] - y[3], dz63 = z[6] - z[3];
-    const T dx20 = x[2] - x[0], dy20 = y[2] - y[0], dz20 = z[2] - z[0];
-    const T dx50 = x[5] - x[0], dy50 = y[5] - y[0], dz50 = z[5] - z[0];
-    const T dx64 = x[6] - x[4], dy64 = y[6] - y[4], dz64 = z[6] - z[4];
-    const T dx31 = x[3] - x[1], dy31 = y[3] - y[1], dz31 = z[3] - z[1];
-    const T dx72 = x[7] - x[2], dy72 = y[7] - y[2], dz72 = z[7] - z[2];
-    const T dx43 = x[4] - x[3], dy43 = y[4] - y[3], dz43 = z[4] - z[3];
-    const T dx57 = x[5] - x[7], dy57 = y[5] - y[7], dz57 = z[5] - z[7];
-    const T dx14 = x[1] - x[4], dy14 = y[1] - y[4], dz14 = z[1] - z[4];
-    const T dx25 = x[2] - x[5], dy25 = y[2] - y[5], dz25 = z[2] - z[5];
-    // grouping of the reference's TRIPLE_PRODUCT macro, lulesh.cc:649-651
-    auto triple = [](T x1, T y1, T z1, T x2, T y2, T z2, T x3, T y3, T z3) {
-        return x1 * (y2 * z3 - z2 * y3) + x2 * (z1 * y3 - y1 * z3) + x3 * (y1 * z2 - z1 * y2);
-    };
-    T volume = triple(dx31 + dx72, dx63, dx20, dy31 + dy72, dy63, dy20, dz31 + dz72, dz63, dz20) +
-               triple(dx43 + dx57, dx64, dx70, dy43 + dy57, dy64, dy70, dz43 + dz57, dz64, dz70) +
-               triple(dx14 + dx25, dx61, dx50, dy14 + dy25, dy61, dy50, dz14 + dz25, dz61, dz50);
-    volume *= twelveth;
-    return volume;
+    LULESH_ELEM_VOLUME_BODY(T)
 }
+template <typename T>
+__device__ __forceinline__ T elem_volume_rounded(const T x[8], const T y[8], const T z[8]) {
+#pragma clang fp contract(off)
+    LULESH_ELEM_VOLUME_BODY(T)
+}
+#undef LULESH_ELEM_VOLUME_BODY
 
 // Jacobian columns and cofactors at the element centre,
 // CalcElemShapeFunctionDerivatives, lulesh.cc:277-374.  b may be null when

@@ -27,6 +27,15 @@
 // in a HIP graph and replayed; kernels of a problem whose loop condition is
 // false exit at once, so the host only polls the time states between graph
 // launches.
+//
+// This file is compiled twice, into two arithmetic forms of the same code:
+// LULESH_IMPL_NS = lulesh_impl_exact with -ffp-contract=off (every multiply
+// and add rounded on its own, in the reference's operand order: bitwise the
+// oracle) and LULESH_IMPL_NS = lulesh_impl_fma with
+// -ffp-contract=fast-honor-pragmas (multiply-adds fused, as the reference's
+// own GCC -O3 -mfma build does; the set-up keeps separate rounding).
+// capi.cc implements the C-ABI and forwards each call to the form its
+// context was created with.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,6 +52,15 @@
 #include "lulesh_elem.hpp"
 #include "lulesh_hip.h"
 #include "lulesh_structured.hpp"
+
+#ifndef LULESH_IMPL_NS
+#define LULESH_IMPL_NS lulesh_impl_exact
+#endif
+
+namespace LULESH_IMPL_NS {
+
+struct lulesh_ctx;
+#include "lulesh_api.inc"
 
 using namespace lulesh_dev;
 
@@ -1607,11 +1625,13 @@ struct Snapshot {
 // ---------------------------------------------------------------------------
 // Sedov set-up in HBM (the Domain constructor, lulesh-init.cc:10-183, single
 // rank, structured cube): the same values the host set-up produces, bit for
-// bit, without the O(E) host loops and the upload.
+// bit, without the O(E) host loops and the upload.  Separate rounding in
+// both arithmetic forms (contract(off) here and in elem_volume_rounded).
 // ---------------------------------------------------------------------------
 // Node coordinates from their lattice indices, lulesh-init.cc:207-227.
 __global__ void k_sedov_nodes(int32_t s, double *__restrict__ x, double *__restrict__ y,
                               double *__restrict__ z) {
+#pragma clang fp contract(off)
     const int n = s + 1;
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n * n * n) return;
@@ -1641,7 +1661,7 @@ __global__ void k_sedov_elems(int32_t s, const double *__restrict__ x,
         yl[a] = y[nd[a]];
         zl[a] = z[nd[a]];
     }
-    const double vol = elem_volume<double>(xl, yl, zl);
+    const double vol = elem_volume_rounded<double>(xl, yl, zl);
     volo[e] = vol;
     elemMass[e] = vol;
     v[e] = 1.0;
@@ -1660,6 +1680,7 @@ __global__ void k_sedov_elems(int32_t s, const double *__restrict__ x,
 // accumulation, lulesh-init.cc:161-166).
 __global__ void k_sedov_nodal_mass(int32_t s, const double *__restrict__ volo,
                                    double *__restrict__ nodalMass) {
+#pragma clang fp contract(off)
     const int n = s + 1;
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n * n * n) return;
@@ -1907,8 +1928,9 @@ int make_group(const std::vector<std::vector<int32_t>> &parts, int nprob, Make &
     return 0;
 }
 
-extern "C" {
-
+// ---------------------------------------------------------------------------
+// Entry points (declared by lulesh_api.inc; capi.cc forwards the C-ABI here)
+// ---------------------------------------------------------------------------
 int lulesh_hip_create(const lulesh_domain_view *v, int precision_opts, lulesh_ctx **out) {
     const lulesh_domain_view *views[1] = {v};
     return create_impl(views, 1, precision_opts, out);
@@ -2383,4 +2405,4 @@ void lulesh_hip_destroy(lulesh_ctx *c) {
     delete c;
 }
 
-} // extern "C"
+} // namespace LULESH_IMPL_NS

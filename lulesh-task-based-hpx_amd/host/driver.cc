@@ -10,6 +10,9 @@
 //   --c <n>   extra cost of expensive regions (1) --q      quiet: CSV line only
 //   --p       progress every 100 cycles
 //   --precision fp64|mixed   arithmetic form (fp64)
+//   --fma     fuse multiply-adds (LULESH_OPT_FMA, as the reference's own
+//             GCC -O3 -mfma build); default: every multiply and add
+//             rounded on its own
 //   --host-loop   TimeIncrement on the host and one lulesh_hip_leapfrog call
 //                 per cycle (the reference's loop shape); default is the
 //                 device-side loop (lulesh_hip_run)
@@ -36,14 +39,14 @@ namespace {
 struct Opts {
     Int_t its = 9999999, nx = 30, numReg = 11, showProg = 0, quiet = 0, balance = 1, cost = 1;
     int precision = LULESH_PRECISION_FP64;
-    bool hostLoop = false, deviceSetup = false;
+    bool hostLoop = false, deviceSetup = false, fma = false;
 };
 
 [[noreturn]] void usage(const char *msg) {
     std::fprintf(stderr, "%s\n", msg);
     std::fprintf(stderr,
                  "usage: lulesh-amd [--s n] [--i n] [--r n] [--b n] [--c n] [--q] [--p]\n"
-                 "                  [--precision fp64|mixed] [--host-loop] [--device-setup]\n");
+                 "                  [--precision fp64|mixed] [--fma] [--host-loop] [--device-setup]\n");
     std::exit(-1);
 }
 
@@ -74,6 +77,7 @@ Opts parse(int argc, char **argv) {
         else if (k == "v") usage("Use of -v requires compiling with -DVIZ_MESH");
         else if (k == "host-loop") o.hostLoop = true;
         else if (k == "device-setup") o.deviceSetup = true;
+        else if (k == "fma") o.fma = true;
         else if (k == "precision") {
             const std::string v = next();
             if (v == "fp64") o.precision = LULESH_PRECISION_FP64;
@@ -135,12 +139,14 @@ int main(int argc, char **argv) {
         std::cout << "Running problem size " << o.nx << "^3 per domain until completion\n";
         std::cout << "Num processors: 1\n";
         std::cout << "Device: one MI355X (gfx950), " << (o.precision ? "mixed" : "fp64")
-                  << " form, " << (o.hostLoop ? "host" : "device") << " time loop\n";
+                  << (o.fma ? " fused multiply-add" : "") << " form, "
+                  << (o.hostLoop ? "host" : "device") << " time loop\n";
         std::cout << "Total number of elements: " << int64_t(o.nx) * o.nx * o.nx << " \n\n";
     }
     if (o.deviceSetup) {
         lulesh_ctx *ctx = nullptr;
-        check(lulesh_hip_create_sedov(o.nx, o.numReg, o.balance, o.cost, o.precision, &ctx),
+        check(lulesh_hip_create_sedov(o.nx, o.numReg, o.balance, o.cost,
+                                      o.precision | (o.fma ? LULESH_OPT_FMA : 0), &ctx),
               ctx);
         lulesh_time_state ts{};
         check(lulesh_hip_get_time_state(ctx, &ts), ctx);
@@ -170,7 +176,7 @@ int main(int argc, char **argv) {
     Domain dom(o.nx, o.numReg, o.balance, o.cost);
     lulesh_domain_view view = dom.view();
     lulesh_ctx *ctx = nullptr;
-    check(lulesh_hip_create(&view, o.precision, &ctx), ctx);
+    check(lulesh_hip_create(&view, o.precision | (o.fma ? LULESH_OPT_FMA : 0), &ctx), ctx);
 
     // Timed window as in the reference (lulesh.cc:2368-2395): the cycle loop
     // only, with the fields already resident on the device.

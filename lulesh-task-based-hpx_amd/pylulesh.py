@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "liblulesh_amd.so")
 
 OK, VOLUME_ERROR, QSTOP_ERROR, INVALID_ARGUMENT = 0, -1, -2, -3
 FP64, MIXED = 0, 1
-OPT_NODE_FORCES, OPT_GENERIC = 0x100, 0x200
+OPT_NODE_FORCES, OPT_GENERIC, OPT_FMA = 0x100, 0x200, 0x400
 
 
 def OPT_STREAMS(k):

@@ -47,3 +47,15 @@ def test_driver_quiet_csv_and_mixed():
     assert fields[:4] == ["20", "11", "50", "1"] and float(fields[5]) > 0
     m = run("--s", "20", "--i", "50", "--precision", "mixed")
     assert "Iteration count     =  50" in m
+
+
+def test_driver_fma_form():
+    """--fma (fused multiply-adds, the reference build's arithmetic): the same
+    report for -s 30, and the set-up in HBM gives the same 17 digits as the
+    host set-up in that form."""
+    f = run("--s", "30", "--fma")
+    assert "Iteration count     =  932" in f
+    assert re.search(r"Final Origin Energy =\s+2\.025075e\+05", f)
+    assert "fused multiply-add" in f
+    fd = run("--s", "30", "--fma", "--device-setup")
+    assert energy17(fd) == energy17(f)
