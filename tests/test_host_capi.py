@@ -123,3 +123,51 @@ def test_driver_flag_errors():
     r = subprocess.run([exe, "--device-setup", "--host-loop"], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 255 and "host set-up" in r.stderr
+
+
+def test_null_context_rejected_by_every_entry_point(pyl):
+    """Every C-ABI entry point taking a context rejects NULL before any
+    device call (the forwarding layer hands NULL to the default form)."""
+    L = pyl.lib()
+    inv = pyl.INVALID_ARGUMENT
+    ts = pyl.TimeState()
+    t = pyl.Tuning()
+    buf = (ctypes.c_double * 4)()
+    assert L.lulesh_hip_leapfrog(None, 1e-6, buf, buf) == inv
+    assert L.lulesh_hip_run(None, 10, ctypes.byref(ts)) == inv
+    assert L.lulesh_hip_phase(None, 0, 1e-6) == inv
+    assert L.lulesh_hip_constraints_result(None, buf, buf) == inv
+    assert L.lulesh_hip_download(None, 0, buf, 4) == inv
+    assert L.lulesh_hip_upload(None, 0, buf, 4) == inv
+    assert L.lulesh_hip_download_view(None, None) == inv
+    assert L.lulesh_hip_set_timing(None, 1) == inv
+    assert L.lulesh_hip_kernel_times(None, None, None, None, 0) == inv
+    assert L.lulesh_hip_set_tuning(None, ctypes.byref(t)) == inv
+    assert L.lulesh_hip_get_tuning(None, ctypes.byref(t)) == inv
+    assert L.lulesh_hip_autotune(None, 4, ctypes.byref(t), buf) == inv
+    assert L.lulesh_hip_batch_run(None, 10, ctypes.byref(ts)) == inv
+    assert L.lulesh_hip_batch_download(None, 0, 0, buf, 4) == inv
+    assert L.lulesh_hip_get_time_state(None, ctypes.byref(ts)) == inv
+    assert L.lulesh_hip_report(None, buf) == inv
+    assert L.lulesh_hip_device_bytes(None) == 0
+    assert L.lulesh_hip_num_problems(None) == 0
+    assert L.lulesh_hip_is_structured(None) == 0
+    assert L.lulesh_hip_last_run_ms(None) == 0.0
+    assert L.lulesh_hip_last_error(None) == b"null context"
+    L.lulesh_hip_destroy(None)
+
+
+def test_fma_form_validates_like_the_default_form(pyl):
+    """LULESH_OPT_FMA selects the fused form; its argument checks are the
+    default form's (no device call is made for invalid arguments)."""
+    L = pyl.lib()
+    h = ctypes.c_void_p()
+    F = pyl.OPT_FMA
+    assert L.lulesh_hip_create(None, F, ctypes.byref(h)) == pyl.INVALID_ARGUMENT
+    dom = pyl.Domain(3)
+    assert L.lulesh_hip_create(ctypes.byref(dom.view), 7 | F, ctypes.byref(h)) == \
+        pyl.INVALID_ARGUMENT
+    assert L.lulesh_hip_create(ctypes.byref(dom.view), F, None) == pyl.INVALID_ARGUMENT
+    for args in [(0, 11, 1, 1, F), (10, 11, 1, 1, pyl.OPT_GENERIC | F), (700, 11, 1, 1, F)]:
+        assert L.lulesh_hip_create_sedov(*args, ctypes.byref(h)) == pyl.INVALID_ARGUMENT, args
+    assert h.value is None
