@@ -176,7 +176,8 @@ int64_t lulesh_hip_device_bytes(const lulesh_ctx *ctx);
 /* 1 if the structured (index-arithmetic, fused z-march) kernels are in use. */
 int32_t lulesh_hip_is_structured(const lulesh_ctx *ctx);
 /* Device milliseconds of the last lulesh_hip_run / lulesh_hip_batch_run
- * (HIP events around the replayed graphs, polling included). */
+ * (HIP events around the replayed graphs, polling included; for a stream
+ * group the slowest stream's). */
 double lulesh_hip_last_run_ms(const lulesh_ctx *ctx);
 
 /* ---------------------------------------------------------------------- */

@@ -39,7 +39,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -2086,16 +2085,15 @@ int lulesh_hip_batch_run(lulesh_ctx *c, int32_t max_cycles, lulesh_time_state *s
         std::vector<std::vector<lulesh_time_state>> st(L);
         std::vector<int> rcs(L, 0);
         for (size_t l = 0; l < L; ++l) st[l].resize(size_t(c->lanes[l]->nprob));
-        const auto t0 = std::chrono::steady_clock::now();
         std::vector<std::thread> th;
         for (size_t l = 1; l < L; ++l)
             th.emplace_back([&, l] { rcs[l] = lulesh_hip_batch_run(c->lanes[l], max_cycles,
                                                                    st[l].data()); });
         rcs[0] = lulesh_hip_batch_run(c->lanes[0], max_cycles, st[0].data());
         for (auto &t : th) t.join();
-        c->last_run_ms =
-            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
-                .count();
+        // the lanes start together: the slowest lane's device time
+        c->last_run_ms = 0;
+        for (lulesh_ctx *l : c->lanes) c->last_run_ms = std::max(c->last_run_ms, l->last_run_ms);
         if (states)
             for (int p = 0; p < c->nprob; ++p)
                 states[p] = st[size_t(c->lane_of[size_t(p)])][size_t(c->local_of[size_t(p)])];
