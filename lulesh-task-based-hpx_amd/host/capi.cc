@@ -33,8 +33,27 @@ namespace fm = lulesh_impl_fma;
 ex::lulesh_ctx *E(const lulesh_ctx *c) { return static_cast<ex::lulesh_ctx *>(c->impl); }
 fm::lulesh_ctx *F(const lulesh_ctx *c) { return static_cast<fm::lulesh_ctx *>(c->impl); }
 
-// Wrap the context a create call of either form produced.
-int wrap(int rc, bool fma, void *impl, lulesh_ctx **out) {
+// Run the create call of the form LULESH_OPT_FMA selects (make_exact or
+// make_fma, given the options without that bit and a pointer of the form's
+// own context type) and wrap the context it produced.
+template <typename MakeExact, typename MakeFma>
+int create_in_form(int precision_opts, lulesh_ctx **out, MakeExact &&make_exact,
+                   MakeFma &&make_fma) {
+    if (!out) return LULESH_INVALID_ARGUMENT;
+    *out = nullptr;
+    const bool fma = (precision_opts & LULESH_OPT_FMA) != 0;
+    const int opts = precision_opts & ~LULESH_OPT_FMA;
+    void *impl = nullptr;
+    int rc;
+    if (fma) {
+        fm::lulesh_ctx *p = nullptr;
+        rc = make_fma(opts, &p);
+        impl = p;
+    } else {
+        ex::lulesh_ctx *p = nullptr;
+        rc = make_exact(opts, &p);
+        impl = p;
+    }
     if (rc != 0 || !impl) return rc;
     *out = new lulesh_ctx{fma, impl};
     return 0;
@@ -51,60 +70,45 @@ int wrap(int rc, bool fma, void *impl, lulesh_ctx **out) {
 extern "C" {
 
 int lulesh_hip_create(const lulesh_domain_view *host, int precision, lulesh_ctx **out) {
-    if (!out) return LULESH_INVALID_ARGUMENT;
-    *out = nullptr;
-    const bool fma = (precision & LULESH_OPT_FMA) != 0;
-    const int opts = precision & ~LULESH_OPT_FMA;
-    void *impl = nullptr;
-    const int rc = fma ? fm::lulesh_hip_create(host, opts, reinterpret_cast<fm::lulesh_ctx **>(&impl))
-                       : ex::lulesh_hip_create(host, opts, reinterpret_cast<ex::lulesh_ctx **>(&impl));
-    return wrap(rc, fma, impl, out);
+    return create_in_form(
+        precision, out,
+        [&](int o, ex::lulesh_ctx **p) { return ex::lulesh_hip_create(host, o, p); },
+        [&](int o, fm::lulesh_ctx **p) { return fm::lulesh_hip_create(host, o, p); });
 }
 
 int lulesh_hip_batch_create(const lulesh_domain_view *const *views, int32_t nprob,
                             int precision, lulesh_ctx **out) {
-    if (!out) return LULESH_INVALID_ARGUMENT;
-    *out = nullptr;
-    const bool fma = (precision & LULESH_OPT_FMA) != 0;
-    const int opts = precision & ~LULESH_OPT_FMA;
-    void *impl = nullptr;
-    const int rc =
-        fma ? fm::lulesh_hip_batch_create(views, nprob, opts,
-                                          reinterpret_cast<fm::lulesh_ctx **>(&impl))
-            : ex::lulesh_hip_batch_create(views, nprob, opts,
-                                          reinterpret_cast<ex::lulesh_ctx **>(&impl));
-    return wrap(rc, fma, impl, out);
+    return create_in_form(
+        precision, out,
+        [&](int o, ex::lulesh_ctx **p) { return ex::lulesh_hip_batch_create(views, nprob, o, p); },
+        [&](int o, fm::lulesh_ctx **p) { return fm::lulesh_hip_batch_create(views, nprob, o, p); });
 }
 
 int lulesh_hip_create_sedov(int32_t s, int32_t nr, int32_t balance, int32_t cost,
                             int precision_opts, lulesh_ctx **out) {
-    if (!out) return LULESH_INVALID_ARGUMENT;
-    *out = nullptr;
-    const bool fma = (precision_opts & LULESH_OPT_FMA) != 0;
-    const int opts = precision_opts & ~LULESH_OPT_FMA;
-    void *impl = nullptr;
-    const int rc =
-        fma ? fm::lulesh_hip_create_sedov(s, nr, balance, cost, opts,
-                                          reinterpret_cast<fm::lulesh_ctx **>(&impl))
-            : ex::lulesh_hip_create_sedov(s, nr, balance, cost, opts,
-                                          reinterpret_cast<ex::lulesh_ctx **>(&impl));
-    return wrap(rc, fma, impl, out);
+    return create_in_form(
+        precision_opts, out,
+        [&](int o, ex::lulesh_ctx **p) {
+            return ex::lulesh_hip_create_sedov(s, nr, balance, cost, o, p);
+        },
+        [&](int o, fm::lulesh_ctx **p) {
+            return fm::lulesh_hip_create_sedov(s, nr, balance, cost, o, p);
+        });
 }
 
 int lulesh_hip_batch_create_sedov(const int32_t *sizes, const int32_t *regions,
                                   const double *factors, int32_t nprob, int32_t balance,
                                   int32_t cost, int precision_opts, lulesh_ctx **out) {
-    if (!out) return LULESH_INVALID_ARGUMENT;
-    *out = nullptr;
-    const bool fma = (precision_opts & LULESH_OPT_FMA) != 0;
-    const int opts = precision_opts & ~LULESH_OPT_FMA;
-    void *impl = nullptr;
-    const int rc =
-        fma ? fm::lulesh_hip_batch_create_sedov(sizes, regions, factors, nprob, balance, cost,
-                                                opts, reinterpret_cast<fm::lulesh_ctx **>(&impl))
-            : ex::lulesh_hip_batch_create_sedov(sizes, regions, factors, nprob, balance, cost,
-                                                opts, reinterpret_cast<ex::lulesh_ctx **>(&impl));
-    return wrap(rc, fma, impl, out);
+    return create_in_form(
+        precision_opts, out,
+        [&](int o, ex::lulesh_ctx **p) {
+            return ex::lulesh_hip_batch_create_sedov(sizes, regions, factors, nprob, balance,
+                                                     cost, o, p);
+        },
+        [&](int o, fm::lulesh_ctx **p) {
+            return fm::lulesh_hip_batch_create_sedov(sizes, regions, factors, nprob, balance,
+                                                     cost, o, p);
+        });
 }
 
 int lulesh_hip_leapfrog(lulesh_ctx *c, double deltatime, double *dtcourant, double *dthydro) {
