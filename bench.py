@@ -167,6 +167,49 @@ def copy_calibration(nbytes, reps):
                       "note": "per copy dispatch: nbytes read, nbytes written"}))
 
 
+def copy_bandwidth(nbytes=1 << 31, reps=10):
+    """Bytes moved per second (read + write) by the HIP runtime's
+    device-to-device copy of `nbytes` (hipMemcpyAsync, its blit kernel),
+    timed with HIP events: a practical HBM ceiling for kernels that stream
+    reads and writes, reported beside the 8 TB/s peak."""
+    import ctypes
+
+    hip = None
+    for name in ("libamdhip64.so.7", "libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+        try:
+            hip = ctypes.CDLL(name)  # already loaded by the library: the same handle
+            break
+        except OSError:
+            continue
+    if hip is None:
+        return None
+    vp = ctypes.c_void_p
+    src, dst, e0, e1 = vp(), vp(), vp(), vp()
+    if hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) or \
+            hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)):
+        return None
+    hip.hipMemset(src, 0, ctypes.c_size_t(nbytes))
+    hip.hipEventCreate(ctypes.byref(e0))
+    hip.hipEventCreate(ctypes.byref(e1))
+    D2D = 3  # hipMemcpyDeviceToDevice
+    for _ in range(3):
+        hip.hipMemcpyAsync(dst, src, ctypes.c_size_t(nbytes), D2D, None)
+    hip.hipEventRecord(e0, None)
+    for _ in range(reps):
+        hip.hipMemcpyAsync(dst, src, ctypes.c_size_t(nbytes), D2D, None)
+    hip.hipEventRecord(e1, None)
+    hip.hipEventSynchronize(e1)
+    ms = ctypes.c_float()
+    hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+    for e in (e0, e1):
+        hip.hipEventDestroy(e)
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    if ms.value <= 0:
+        return None
+    return 2 * nbytes * reps / (ms.value * 1e-3) / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,6 +237,8 @@ def main():
                     help="autotune the s=150 legs first (0: the library's default launch shapes)")
     ap.add_argument("--batch-tune-cycles", type=int, default=6,
                     help="autotune the batch leg first (0: default launch shapes)")
+    ap.add_argument("--no-copy-reference", action="store_true",
+                    help="skip the 2 GiB device-copy bandwidth reference")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -367,6 +412,7 @@ def main():
                          "sample": f"serial oracle, s={cs}, {nr} regions, {ncyc1} cycles after "
                                    f"cycle 1, {el1:.1f}s wall"}
 
+    copy_gbs = None if args.no_copy_reference else copy_bandwidth()
     bsz = model_bytes_per_zone_cycle(s)
     out = {
         "metric": "zone-cycles/sec FP64 s=150 vs FP32-mixed & 64-problem batch; % HBM peak",
@@ -397,6 +443,7 @@ def main():
                      "fp64_vector": fp64,
                      "source_of_traffic_and_valu_busy": TRAFFIC_FILES[args.arith]
                      if traffic else None},
+        "hbm_copy_reference_GBs": round(copy_gbs, 1) if copy_gbs else None,
         "model_bytes_per_zone_cycle": round(bsz, 1),
         "model_hbm_frac": round(bsz * zcps / (HBM_PEAK_GBS * 1e9), 4),
         "kernels": kernels,
