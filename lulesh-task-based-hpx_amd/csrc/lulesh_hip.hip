@@ -783,7 +783,8 @@ struct lulesh_ctx {
     bool structured = false;
     bool store_forces = false; // also write fx.. and xdd.. (Domain fidelity)
     int nprob = 0;
-    int cus = 256; // compute units of the device
+    int cus = 256;         // compute units of the device
+    bool zSquare16 = true; // every problem's z-march patch is 16 x 16
     std::vector<ProbHost> probs;
     int64_t totE = 0, totN = 0;
     Params P{};
@@ -932,7 +933,9 @@ int auto_zplanes(const lulesh_ctx *c) {
         // model's pick was the fastest of every group count measured at
         // s = 150 and s = 250.
         const int n = c->probs[0].s + 1;
-        const int64_t t2 = int64_t(zmarch_tiles(n)) * zmarch_tiles(n);
+        int pw, ph;
+        zmarch_shape(n, pw, ph);
+        const int64_t t2 = int64_t(zmarch_tiles(n, pw)) * zmarch_tiles(n, ph);
         const int64_t slots = 2 * int64_t(std::max(c->cus, 1));
         bool any_full = false;
         for (int g = 1; g <= n && !any_full; ++g) any_full = 10 * t2 * g >= 9 * slots;
@@ -955,8 +958,10 @@ int auto_zplanes(const lulesh_ctx *c) {
     auto blocks = [&](int z) {
         int64_t b = 0;
         for (const auto &p : c->probs) {
-            const int n = p.s + 1, t = zmarch_tiles(n);
-            b += int64_t(t) * t * ((n + z - 1) / z);
+            const int n = p.s + 1;
+            int pw, ph;
+            zmarch_shape(n, pw, ph);
+            b += int64_t(zmarch_tiles(n, pw)) * zmarch_tiles(n, ph) * ((n + z - 1) / z);
         }
         return b;
     };
@@ -977,6 +982,7 @@ int configure(lulesh_ctx *c) {
     if (t.material_mode == 1 && !c->matList) t.material_mode = 0; // no region lists
     std::vector<int32_t> zb, eb, mb, ob;
     c->hpd.clear();
+    c->zSquare16 = true;
     c->hsegs.clear();
     for (int pi = 0; pi < c->nprob; ++pi) {
         const ProbHost &ph = c->probs[size_t(pi)];
@@ -987,12 +993,15 @@ int configure(lulesh_ctx *c) {
         d.elemOff = ph.elemOff;
         d.nodeOff = ph.nodeOff;
         const int n = ph.s + 1;
-        d.tiles = zmarch_tiles(n);
+        zmarch_shape(n, d.pw, d.ph);
+        c->zSquare16 = c->zSquare16 && d.pw == 16 && d.ph == 16;
+        d.tilesX = zmarch_tiles(n, d.pw);
+        d.tilesY = zmarch_tiles(n, d.ph);
         d.inv_s = 1.0 / double(ph.s);
         d.inv_ss = 1.0 / (double(ph.s) * double(ph.s));
         d.zplanes = std::min(t.zplanes, n);
         d.zBlock0 = int32_t(zb.size());
-        const int64_t nz = int64_t(d.tiles) * d.tiles * ((n + d.zplanes - 1) / d.zplanes);
+        const int64_t nz = int64_t(d.tilesX) * d.tilesY * ((n + d.zplanes - 1) / d.zplanes);
         zb.insert(zb.end(), size_t(nz), pi);
         d.eBlock0 = int32_t(eb.size());
         eb.insert(eb.end(), size_t(grid_for(ph.E, t.block_elem)), pi);
@@ -1045,7 +1054,11 @@ int enqueue_nodal(lulesh_ctx *c) {
         const NodeForceOut fo{c->f[LULESH_F_FX], c->f[LULESH_F_FY], c->f[LULESH_F_FZ],
                               c->f[LULESH_F_XDD], c->f[LULESH_F_YDD], c->f[LULESH_F_ZDD]};
         return launch(c, T_NODAL, [&] {
-            auto kern = c->store_forces ? k_nodal_zmarch<R, true> : k_nodal_zmarch<R, false>;
+            auto kern = c->zSquare16
+                            ? (c->store_forces ? k_nodal_zmarch<R, true, true>
+                                               : k_nodal_zmarch<R, false, true>)
+                            : (c->store_forces ? k_nodal_zmarch<R, true, false>
+                                               : k_nodal_zmarch<R, false, false>);
             kern<<<c->zBlocks, kZmarchThreads, 0, c->stream>>>(
                 c->d_pd, c->d_zBlockProb, c->nb, fo, c->f[LULESH_F_P], c->f[LULESH_F_Q],
                 c->f[LULESH_F_VOLO], c->f[LULESH_F_V], c->f[LULESH_F_SS], c->f[LULESH_F_ELEMMASS],

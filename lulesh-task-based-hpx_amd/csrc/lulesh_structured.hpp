@@ -9,13 +9,15 @@
 // k_nodal_zmarch fuses the whole nodal phase (lulesh.cc:2017-2110) into one
 // pass without storing corner forces in HBM.  A workgroup owns a tile of
 // node columns over a range of node planes and marches upward in z: for
-// each element plane k its 16x16 threads compute the stress and hourglass
-// corner forces of the 16x16 element patch covering the tile, stage them in
-// LDS, and the owning threads add them to their nodes.  Along x (and y) the
-// first tile owns node columns 0..15 of patch element columns 0..15; tile
-// t >= 1 owns the 15 node columns 15t+1 .. 15t+15 and its patch starts one
-// element lower (15t), the halo its first node column needs.  The node
-// forces are summed in the reference's corner order
+// each element plane k its threads (one per element) compute the stress and
+// hourglass corner forces of the pw x ph element patch covering the tile
+// (16 x 16 unless another shape needs fewer tiles: zmarch_shape), stage them
+// in LDS, and the owning threads add them to their nodes.  Along x the first
+// tile owns node columns 0..pw-1 of patch element columns 0..pw-1; tile
+// t >= 1 owns the pw-1 node columns (pw-1)t+1 .. (pw-1)t+pw-1 and its patch
+// starts one element lower ((pw-1)t), the halo its first node column needs;
+// likewise along y with ph.  The node forces are summed in the reference's
+// corner order
 // (ascending element index, stress then hourglass per corner,
 // lulesh.cc:849-857):
 //   node plane k+1 starts with corners 6,7,5,4 of element plane k,
@@ -30,14 +32,39 @@
 
 namespace lulesh_dev {
 
-constexpr int kPatch = 16;            // element patch edge (threads per tile edge)
-constexpr int kTile = kPatch - 1;     // node columns owned by a tile after the first
-constexpr int kZmarchThreads = kPatch * kPatch;
-constexpr int kPatchNodes = (kPatch + 1) * (kPatch + 1); // node columns of a patch
+constexpr int kZmarchThreads = 256;  // threads per z-march workgroup
+constexpr int kPatchNodes = 17 * 17; // LDS node slots per plane: (pw + 1) (ph + 1) at most
 
-// Tiles along an edge of n nodes: 16 node columns, then 15 per tile.
-__host__ __device__ inline int zmarch_tiles(int n) {
-    return n <= kPatch ? 1 : 1 + (n - kPatch + kTile - 1) / kTile;
+// Tiles along an edge of n nodes for patches of P elements: P node columns
+// for the first tile, P - 1 for each further one.
+__host__ __device__ inline int zmarch_tiles(int n, int P) {
+    return n <= P ? 1 : 1 + (n - P + P - 2) / (P - 1);
+}
+
+// Patch shape (pw x ph elements) of one problem.  16 x 16 unless another
+// shape that fits the workgroup (pw ph <= 256 threads) and the LDS node
+// ring ((pw + 1)(ph + 1) <= 289) needs at least 3% fewer tiles per plane
+// group; among those the fewest tiles, then more threads, then the wider
+// patch (longer coalesced rows).  s = 150 keeps 16 x 16 (10 x 10 tiles);
+// small meshes gain most (s = 20: two 23 x 11 tiles instead of four).
+inline void zmarch_shape(int n, int &pw, int &ph) {
+    const int t16 = zmarch_tiles(n, 16) * zmarch_tiles(n, 16);
+    int best = 0, bestThreads = 0, bw = 16, bh = 16;
+    for (int w = 8; w <= 32; ++w)
+        for (int h = 8; h <= 32; ++h) {
+            if (w * h > kZmarchThreads || (w + 1) * (h + 1) > kPatchNodes) continue;
+            const int t = zmarch_tiles(n, w) * zmarch_tiles(n, h);
+            if (best == 0 || t < best || (t == best && w * h > bestThreads) ||
+                (t == best && w * h == bestThreads && w > bw)) {
+                best = t;
+                bestThreads = w * h;
+                bw = w;
+                bh = h;
+            }
+        }
+    const bool take = 100 * best <= 97 * t16;
+    pw = take ? bw : 16;
+    ph = take ? bh : 16;
 }
 
 struct NodeIn {
@@ -55,7 +82,9 @@ struct NodeForceOut { // optional copies of fx.. / xdd.. (Domain fidelity)
 struct ProbDesc {
     int32_t s, E, N;
     int32_t elemOff, nodeOff; // offsets into the concatenated element / node arrays
-    int32_t zplanes, tiles;   // z-march: node planes per workgroup, tiles per edge
+    int32_t zplanes;          // z-march: node planes per workgroup
+    int32_t pw, ph;           // z-march patch: elements per row, rows
+    int32_t tilesX, tilesY;   // z-march tiles along x and y
     int32_t zBlock0, eBlock0; // first workgroup of this problem in the z-march / element grids
     int32_t mBlock0, pad;     // first workgroup in the element-order material grid
     // 1/s and 1/s^2 rounded to double: element (k, j, i) from its index li
@@ -68,7 +97,9 @@ struct ProbDesc {
 // Two waves per SIMD: the FP64 form needs about 260 VGPRs with the register
 // prefetch of the next node plane; holding it to 256 costs a handful of
 // spilled values and keeps two workgroups per CU.
-template <typename HG, bool STORE_F>
+// SQUARE16: every problem of the launch has 16 x 16 patches (the shape is
+// then a compile-time constant; runtime shapes cost 2-6% at s=150).
+template <typename HG, bool STORE_F, bool SQUARE16>
 __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
     const ProbDesc *__restrict__ pds, const int32_t *__restrict__ blockProb, NodeBufs nb,
     NodeForceOut fo0, const double *__restrict__ p0,
@@ -100,36 +131,37 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
     const double *__restrict__ nodalMass = nodalMass0 + no;
     const int lb = blk - pd.zBlock0;
     // LDS: 4 corners x (3 stress + 3 hourglass) values per element of the
-    // patch, and the 17x17 patch nodes of the two node planes an element
-    // plane touches (a ring of two planes: x, y, z, xd, yd, zd each).
+    // patch, and the (pw+1) x (ph+1) patch nodes of the two node planes an
+    // element plane touches (a ring of two planes: x, y, z, xd, yd, zd each).
     __shared__ double lds[24][kZmarchThreads];
     __shared__ double pn[2][6][kPatchNodes];
 
     const int n = s + 1;
     const int tid = threadIdx.x;
-    const int jj = tid / kPatch, ii = tid % kPatch;
-    const int tilesX = pd.tiles;
-    const int tile = lb % (tilesX * tilesX);
+    const int pw = SQUARE16 ? 16 : pd.pw, ph = SQUARE16 ? 16 : pd.ph; // patch: pw x ph elements
+    const int jj = tid / pw, ii = tid - jj * pw; // (threads past pw * ph idle: jj >= ph)
+    const int tilesX = pd.tilesX, tilesXY = pd.tilesX * pd.tilesY;
+    const int tile = lb % tilesXY;
     const int ty = tile / tilesX, tx = tile % tilesX;
-    const int dy = ty > 0, dx = tx > 0;           // 1: the patch has a low-side halo
-    const int E0y = ty * kTile, E0x = tx * kTile; // first element row / column of the patch
-    const int K0 = (lb / (tilesX * tilesX)) * planes_per_block;
+    const int dy = ty > 0, dx = tx > 0;                 // 1: the patch has a low-side halo
+    const int E0y = ty * (ph - 1), E0x = tx * (pw - 1); // first element row / column
+    const int K0 = (lb / tilesXY) * planes_per_block;
     const int K1 = min(K0 + planes_per_block, n); // owned node planes [K0, K1)
     const int ej = E0y + jj, ei = E0x + ii;        // this thread's element column
-    const bool elemCol = (ej < s) && (ei < s);
+    const bool elemCol = (jj < ph) && (ej < s) && (ei < s);
     const int J = E0y + dy + jj, I = E0x + dx + ii; // this thread's node column
-    const bool ownsNode = (jj < kPatch - dy) && (ii < kPatch - dx) && (J < n) && (I < n);
+    const bool ownsNode = (jj < ph - dy) && (ii < pw - dx) && (J < n) && (I < n);
     // validity of the four elements around node (J,I) in a plane:
-    // (J-1,I-1) (J-1,I) (J,I-1) (J,I)  <->  patch threads tb, tb+1, tb+16, tb+17
+    // (J-1,I-1) (J-1,I) (J,I-1) (J,I)  <->  patch threads tb, tb+1, tb+pw, tb+pw+1
     // (tb < 0 only where J = 0 or I = 0, whose elements are not read)
     const bool v00 = (J >= 1) && (I >= 1), v01 = (J >= 1) && (I < s), v10 = (J < s) && (I >= 1),
                v11 = (J < s) && (I < s);
-    const int tb = tid + (dy - 1) * kPatch + (dx - 1);
+    const int tb = tid + (dy - 1) * pw + (dx - 1);
     const double dt = st->deltatime;
 
     double px = 0.0, py = 0.0, pz = 0.0; // partial sums of the node in the current plane
 
-    const int own = (jj + dy) * (kPatch + 1) + (ii + dx); // node (J, I) in the patch node array
+    const int own = (jj + dy) * (pw + 1) + (ii + dx); // node (J, I) in the patch node array
     auto finish_node = [&](int K, double mass) {
         const int g = (K * n + J) * n + I;
         const int slot = K & 1;
@@ -171,9 +203,9 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
         const int t = tid + m * kZmarchThreads;
-        const int a = t / (kPatch + 1), b = t - a * (kPatch + 1);
+        const int a = t / (pw + 1), b = t - a * (pw + 1);
         const int Jn = E0y + a, In = E0x + b;
-        pin[m] = t < kPatchNodes && Jn < n && In < n;
+        pin[m] = t < (pw + 1) * (ph + 1) && Jn < n && In < n;
         pg[m] = Jn * n + In;
     }
     double rn[2][6];
@@ -268,8 +300,8 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
             const double determ = ev[2] * ev[3];
             // the element's nodes in the patch node arrays (reference local
             // order, lulesh-init.cc:237-244): bottom plane k, top plane k+1
-            const int q0 = jj * (kPatch + 1) + ii;
-            const int nq[4] = {q0, q0 + 1, q0 + kPatch + 2, q0 + kPatch + 1};
+            const int q0 = jj * (pw + 1) + ii;
+            const int nq[4] = {q0, q0 + 1, q0 + pw + 2, q0 + pw + 1};
             const int sb = k & 1, st1 = sb ^ 1;
             double x[8], y[8], z[8];
 #pragma unroll
@@ -358,8 +390,8 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
         if (ownsNode && k >= K0) {
             if (v00) add(2, tb);
             if (v01) add(3, tb + 1);
-            if (v10) add(1, tb + kPatch);
-            if (v11) add(0, tb + kPatch + 1);
+            if (v10) add(1, tb + pw);
+            if (v11) add(0, tb + pw + 1);
             finish_node(k, mcur);
         }
         __syncthreads();
@@ -372,10 +404,10 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
         // lds now holds corners 4..7 at slots 0..3
         px = py = pz = 0.0;
         if (ownsNode && k + 1 < K1) {
-            if (v00) add(2, tb);              // corner 6 of element (J-1, I-1)
-            if (v01) add(3, tb + 1);          // corner 7 of element (J-1, I)
-            if (v10) add(1, tb + kPatch);     // corner 5 of element (J, I-1)
-            if (v11) add(0, tb + kPatch + 1); // corner 4 of element (J, I)
+            if (v00) add(2, tb);          // corner 6 of element (J-1, I-1)
+            if (v01) add(3, tb + 1);      // corner 7 of element (J-1, I)
+            if (v10) add(1, tb + pw);     // corner 5 of element (J, I-1)
+            if (v11) add(0, tb + pw + 1); // corner 4 of element (J, I)
         }
         mcur = mnext;
     }
