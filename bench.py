@@ -5,10 +5,18 @@ Workload: plain FP64 Sedov blast at s=150 (3,375,000 elements, 151^3 nodes,
 A step is one cycle (device TimeIncrement + LagrangeLeapFrog) over the whole
 mesh; the fields are resident in HBM before the timed region starts.
 
+Every line runs in one arithmetic form (--arith): "fma" (default:
+multiply-adds fused, as the reference's own GCC -O3 -mfma build) or "exact"
+(every multiply and add rounded on its own: bitwise the oracle); the s=150
+FP64 line of the other form is measured beside the headline, with its
+origin-energy difference from it.
+
 Also reported on the same GPU: the FP32-mixed form at s=150 (with its
-origin-energy error against the FP64 path after the same cycles), per-kernel
-HIP-event timings and the dominant kernel's roofline, and the CPU oracle
-(OpenMP port of the reference arithmetic) timed on this host's cores.
+origin-energy error against the exact FP64 form after the same cycles), the
+s=250 / 21-region problem (FP64 and mixed), the 64-problem batch, per-kernel
+HIP-event timings and the dominant kernel's roofline, a device-copy
+bandwidth reference, and the CPU oracle (OpenMP port of the reference
+arithmetic) timed on this host's cores.
 
 Prints one JSON line.
 """
