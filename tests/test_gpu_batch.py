@@ -81,18 +81,20 @@ def test_batch_per_problem_stop_time(pyl):
     assert len(cycles) > 1
 
 
-def test_autotune_keeps_state(pyl):
+@pytest.mark.parametrize("fma", [False, True])
+def test_autotune_keeps_state(pyl, fma):
     """The autotuner restores the state it started from: a run after tuning
-    equals a run without it, bit for bit."""
-    banner("autotune")
+    equals a run without it, bit for bit (both arithmetic forms)."""
+    banner(f"autotune fma={fma}")
+    opts = pyl.OPT_FMA if fma else 0
     params = [(10, 11, 1.0), (14, 16, 1.2)]
-    a = pyl.Batch([pyl.make_domain(*p) for p in params])
+    a = pyl.Batch([pyl.make_domain(*p) for p in params], opts=opts)
     rc, _ = a.run(20)
     assert rc == 0
     t, ms = a.autotune(6)
     assert t.zplanes >= 1 and t.block_elem in (64, 128, 256) and ms > 0
     rc, sa = a.run(50)
-    b = pyl.Batch([pyl.make_domain(*p) for p in params])
+    b = pyl.Batch([pyl.make_domain(*p) for p in params], opts=opts)
     rc2, sb = b.run(50)
     assert rc == rc2 == 0
     for k in range(2):

@@ -8,8 +8,9 @@ scale).
   oracle's; the first two cycles bitwise (no cbrt-dependent term reaches the
   state before cycle 3).
 * s=250: the first two cycles bitwise against the oracle, then 500 cycles of
-  FP64 and of the FP32-mixed form: mixed origin energy within 1e-6 of FP64,
-  and the FP64 symmetry diffs at rounding level.
+  FP64, of the fused-multiply-add FP64 form (origin energy within 1e-9) and
+  of the FP32-mixed form (within 1e-6), and the FP64 symmetry diffs at
+  rounding level.
 """
 import os
 
@@ -64,11 +65,21 @@ def test_s150_300_cycles_vs_oracle(pyl):
 
 
 def test_s250_fp64_and_mixed(pyl):
-    banner("s=250, 21 regions: 500 cycles FP64 and mixed")
+    banner("s=250, 21 regions: 500 cycles FP64, fused FP64 and mixed")
     d64, dev64, t64 = pyl.run_device(250, 500, 21)
     rep64 = d64.report()
     e64 = d64.field("e")[0]
     dev64.close()
+    # the fused arithmetic form, built in HBM (its set-up equals the host one)
+    df = pyl.Device.sedov(250, 21, opts=pyl.OPT_FMA)
+    tf = df.time_state()
+    assert df.run(500, tf) == 0 and tf.cycle == 500
+    repf = df.report()
+    df.close()
+    print(f"[fullsize] fused e0 rel to fp64 {abs(repf[0] - e64) / abs(e64):.3e}, "
+          f"MaxRelDiff {repf[3]:.3e}", flush=True)
+    assert abs(repf[0] - e64) <= 1e-9 * abs(e64)
+    assert repf[3] <= 1e-10
     dm, devm, tm = pyl.run_device(250, 500, 21, precision=pyl.MIXED)
     repm = dm.report()
     em = dm.field("e")[0]
