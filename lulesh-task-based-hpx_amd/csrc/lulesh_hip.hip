@@ -977,7 +977,7 @@ int configure(lulesh_ctx *c) {
     if (t.block_material != 64 && t.block_material != 128 && t.block_material != 256)
         t.block_material = 128;
     if (t.zplanes <= 0) t.zplanes = auto_zplanes(c);
-    if (t.cycles_per_graph <= 0) t.cycles_per_graph = c->totE >= 1000000 ? 8 : 32;
+    if (t.cycles_per_graph <= 0) t.cycles_per_graph = c->totE >= 1000000 ? 16 : 32;
     if (t.material_mode != 1 || !c->erep) t.material_mode = c->erep ? 0 : 1;
     if (t.material_mode == 1 && !c->matList) t.material_mode = 0; // no region lists
     std::vector<int32_t> zb, eb, mb, ob;
