@@ -938,14 +938,13 @@ int auto_zplanes(const lulesh_ctx *c) {
         const int64_t t2 = int64_t(zmarch_tiles(n, pw)) * zmarch_tiles(n, ph);
         const int64_t slots = 2 * int64_t(std::max(c->cus, 1));
         bool any_full = false;
-        for (int g = 1; g <= n && !any_full; ++g) any_full = 10 * t2 * g >= 9 * slots;
+        for (int z = 1; z <= n && !any_full; ++z) any_full = 10 * t2 * zmarch_groups(n, z) >= 9 * slots;
         int best = 1;
         int64_t best_cost = INT64_MAX;
-        for (int g = 1; g <= n; ++g) {
-            const int z = (n + g - 1) / g;
-            const int64_t wgs = t2 * ((n + z - 1) / z);
+        for (int z = n; z >= 1; --z) { // ties: the smaller z (more workgroups)
+            const int64_t wgs = t2 * zmarch_groups(n, z);
             if (any_full && 10 * wgs < 9 * slots) continue;
-            const int64_t cost = ((wgs + slots - 1) / slots) * (z + 1);
+            const int64_t cost = ((wgs + slots - 1) / slots) * zmarch_steps(n, z);
             if (cost <= best_cost) {
                 best_cost = cost;
                 best = z;
@@ -961,7 +960,7 @@ int auto_zplanes(const lulesh_ctx *c) {
             const int n = p.s + 1;
             int pw, ph;
             zmarch_shape(n, pw, ph);
-            b += int64_t(zmarch_tiles(n, pw)) * zmarch_tiles(n, ph) * ((n + z - 1) / z);
+            b += int64_t(zmarch_tiles(n, pw)) * zmarch_tiles(n, ph) * zmarch_groups(n, z);
         }
         return b;
     };
@@ -1001,7 +1000,7 @@ int configure(lulesh_ctx *c) {
         d.inv_ss = 1.0 / (double(ph.s) * double(ph.s));
         d.zplanes = std::min(t.zplanes, n);
         d.zBlock0 = int32_t(zb.size());
-        const int64_t nz = int64_t(d.tilesX) * d.tilesY * ((n + d.zplanes - 1) / d.zplanes);
+        const int64_t nz = int64_t(d.tilesX) * d.tilesY * zmarch_groups(n, d.zplanes);
         zb.insert(zb.end(), size_t(nz), pi);
         d.eBlock0 = int32_t(eb.size());
         eb.insert(eb.end(), size_t(grid_for(ph.E, t.block_elem)), pi);
@@ -2296,7 +2295,7 @@ int lulesh_hip_autotune(lulesh_ctx *c, int32_t cycles, lulesh_tuning *best,
         for (const auto &p : x->probs) nmax = std::max(nmax, p.s + 1);
     std::vector<int> zcand;
     for (int g = 40; g >= 2; --g) {
-        const int z = (nmax + g - 1) / g;
+        const int z = (nmax + 2 * g - 2) / g; // the smallest z with at most g plane groups
         if (z >= 2 && (zcand.empty() || zcand.back() != z)) zcand.push_back(z);
     }
     const int bcand[] = {64, 128, 256};

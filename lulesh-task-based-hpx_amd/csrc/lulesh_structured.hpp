@@ -28,6 +28,8 @@
 // neighbouring tiles still read the old positions.
 #pragma once
 
+#include <algorithm>
+
 #include "lulesh_elem.hpp"
 
 namespace lulesh_dev {
@@ -39,6 +41,27 @@ constexpr int kPatchNodes = 17 * 17; // LDS node slots per plane: (pw + 1) (ph +
 // for the first tile, P - 1 for each further one.
 __host__ __device__ inline int zmarch_tiles(int n, int P) {
     return n <= P ? 1 : 1 + (n - P + P - 2) / (P - 1);
+}
+
+// Plane groups of the z-march (z node planes per workgroup): group 0 owns
+// node planes [0, z), every later group z - 1 planes, because it also
+// recomputes the element plane below its first node plane; so every
+// workgroup walks at most z element planes.  z = 1: one plane per group.
+__host__ __device__ inline int zmarch_group_start(int g, int z) {
+    return z <= 1 ? g : (g == 0 ? 0 : z + (g - 1) * (z - 1));
+}
+__host__ __device__ inline int zmarch_groups(int n, int z) {
+    if (z <= 1) return n;
+    return n <= z ? 1 : 1 + (n - z + z - 2) / (z - 1);
+}
+// Element planes the longest plane group of an edge of n nodes walks.
+inline int zmarch_steps(int n, int z) {
+    int m = 0;
+    for (int g = 0, G = zmarch_groups(n, z); g < G; ++g) {
+        const int K0 = zmarch_group_start(g, z), K1 = std::min(zmarch_group_start(g + 1, z), n);
+        m = std::max(m, std::min(K1, n - 1) - std::max(K0 - 1, 0));
+    }
+    return m;
 }
 
 // Patch shape (pw x ph elements) of one problem.  16 x 16 unless another
@@ -82,7 +105,7 @@ struct NodeForceOut { // optional copies of fx.. / xdd.. (Domain fidelity)
 struct ProbDesc {
     int32_t s, E, N;
     int32_t elemOff, nodeOff; // offsets into the concatenated element / node arrays
-    int32_t zplanes;          // z-march: node planes per workgroup
+    int32_t zplanes;          // z-march: node planes of the first plane group (zmarch_groups)
     int32_t pw, ph;           // z-march patch: elements per row, rows
     int32_t tilesX, tilesY;   // z-march tiles along x and y
     int32_t zBlock0, eBlock0; // first workgroup of this problem in the z-march / element grids
@@ -145,8 +168,9 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
     const int ty = tile / tilesX, tx = tile % tilesX;
     const int dy = ty > 0, dx = tx > 0;                 // 1: the patch has a low-side halo
     const int E0y = ty * (ph - 1), E0x = tx * (pw - 1); // first element row / column
-    const int K0 = (lb / tilesXY) * planes_per_block;
-    const int K1 = min(K0 + planes_per_block, n); // owned node planes [K0, K1)
+    const int zg = lb / tilesXY; // plane group
+    const int K0 = zmarch_group_start(zg, planes_per_block);
+    const int K1 = min(zmarch_group_start(zg + 1, planes_per_block), n); // owned planes [K0, K1)
     const int ej = E0y + jj, ei = E0x + ii;        // this thread's element column
     const bool elemCol = (jj < ph) && (ej < s) && (ei < s);
     const int J = E0y + dy + jj, I = E0x + dx + ii; // this thread's node column
