@@ -204,6 +204,34 @@ def test_structured_equals_generic_and_mixed_loops(pyl):
         np.testing.assert_array_equal(dom.field(f), a.field(f), err_msg=f)
 
 
+@pytest.mark.parametrize("s", [15, 16, 20, 23, 30, 33, 52])
+def test_zmarch_geometries_equal_generic(pyl, s):
+    """The z-march's tiles, halos and plane groups reach every node once, in
+    the reference's corner order, for each patch shape the library picks
+    (s=15: one full 16x16 tile; 16: 17x15; 20: 23x11; 23: 25x10; 30: 2x2
+    tiles of 16x16; 33: 21x12; 52: 27x9 in 2x7 tiles) and for 1, 2, 3 and 7
+    node planes per workgroup as well as the default: every state field
+    equals the indexed path's (corner-force CSR, no z-march) after 30
+    cycles, bit for bit."""
+    banner(f"z-march geometries s={s}")
+    ref, _, tr = pyl.run_device(s, 30, opts=pyl.OPT_GENERIC)
+    assert tr.cycle == 30
+    for z in (1, 2, 3, 7, 0):
+        dom = pyl.Domain(s)
+        dev = pyl.Device(dom)
+        assert dev.structured()
+        t = dev.tuning()
+        t.zplanes = z
+        dev.set_tuning(t)
+        ts = dom.time_state()
+        assert dev.run(30, ts) == 0 and ts.cycle == 30
+        dev.download_view()
+        for f in STATE:
+            np.testing.assert_array_equal(dom.field(f), ref.field(f), err_msg=f"{f} z={z}")
+        dev.close()
+        dom.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("path", ["structured", "generic"])
 def test_material_modes_agree(pyl, precision, path):
