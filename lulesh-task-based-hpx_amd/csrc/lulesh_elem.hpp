@@ -97,6 +97,16 @@ __device__ __forceinline__ float sqrt_area(float x) { return sqrt_(x); }
 // asm statement (no instruction is emitted).
 __device__ __forceinline__ void opaque(double &v) { asm volatile("" : "+v"(v)); }
 
+// Store of a per-element result.  NT: with the non-temporal hint (`nt` on
+// the store), so the written lines are not kept in the L2 for reuse.
+template <bool NT, typename T>
+__device__ __forceinline__ void stream_put(T *p, T v) {
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
 // CalcElemVolume, lulesh.cc:592-674.  The expression is written once and
 // used twice: elem_volume follows the translation unit's contraction setting
 // (the cycle's arithmetic form), elem_volume_rounded rounds every multiply

@@ -354,13 +354,14 @@ __global__ __launch_bounds__(kBlock) void k_nodes(
 // (lulesh.cc:1277-1418).  Gradients are FP64, or FP32 in the mixed form.
 // STRUCT: node indices from the element's (k,j,i); else from nodelist.
 // The end-of-step node state is the half written by the nodal phase.
+// NT: the results are stored with the non-temporal hint (stream_put).
 // ---------------------------------------------------------------------------
 struct ElemKin {
     const double *volo;
     double *v, *vnew, *delv, *arealg, *vdov;
 };
 
-template <typename QG, bool STRUCT>
+template <typename QG, bool STRUCT, bool NT>
 __global__ __launch_bounds__(kBlock) void k_elems(
     const ProbDesc *__restrict__ pds, const int32_t *__restrict__ blockProb,
     const int32_t *__restrict__ nodelist, NodeBufs nb, ElemKin f0, QG *__restrict__ dvxi,
@@ -402,9 +403,9 @@ __global__ __launch_bounds__(kBlock) void k_elems(
     const double vold = f0.v[i];
     const double volume = elem_volume<double>(x, y, z);
     const double rel = volume / voloi;
-    f0.vnew[i] = rel;
-    f0.delv[i] = rel - vold;
-    f0.arealg[i] = characteristic_length(x, y, z, volume);
+    stream_put<NT>(&f0.vnew[i], rel);
+    stream_put<NT>(&f0.delv[i], rel - vold);
+    stream_put<NT>(&f0.arealg[i], characteristic_length(x, y, z, volume));
     // Q gradients on the end-of-step geometry (lulesh.cc:1349)
     {
         QG g[6];
@@ -423,12 +424,12 @@ __global__ __launch_bounds__(kBlock) void k_elems(
             }
             q_gradients<float>(xf, yf, zf, xv, yv, zv, float(voloi * rel), g);
         }
-        dxxi[i] = g[0];
-        dxeta[i] = g[1];
-        dxzeta[i] = g[2];
-        dvxi[i] = g[3];
-        dveta[i] = g[4];
-        dvzeta[i] = g[5];
+        stream_put<NT>(&dxxi[i], g[0]);
+        stream_put<NT>(&dxeta[i], g[1]);
+        stream_put<NT>(&dxzeta[i], g[2]);
+        stream_put<NT>(&dvxi[i], g[3]);
+        stream_put<NT>(&dveta[i], g[4]);
+        stream_put<NT>(&dvzeta[i], g[5]);
     }
     // Velocity divergence on the half-step geometry (lulesh.cc:1218-1239)
     const double dt2 = 0.5 * deltaTime;
@@ -440,13 +441,13 @@ __global__ __launch_bounds__(kBlock) void k_elems(
     }
     double B[3][8];
     const double detJ = shape_derivatives<true>(x, y, z, B);
-    f0.vdov[i] = velocity_divergence(xd, yd, zd, B, detJ);
+    stream_put<NT>(&f0.vdov[i], velocity_divergence(xd, yd, zd, B, detJ));
     // CheckAndUpdateVolumeForElems, lulesh.cc:1255-1270
     double vc = vold;
     if (eosvmin != 0.0 && vc < eosvmin) vc = eosvmin;
     if (eosvmax != 0.0 && vc > eosvmax) vc = eosvmax;
     if (vc < 0.0) st->error = kVolumeError;
-    f0.v[i] = (fabs_(rel - 1.0) < v_cut) ? 1.0 : rel;
+    stream_put<NT>(&f0.v[i], (fabs_(rel - 1.0) < v_cut) ? 1.0 : rel);
 }
 
 struct ElemFields {
@@ -580,12 +581,13 @@ __global__ __launch_bounds__(kBlock) void k_material(
 // evaluation reads its inputs from LDS (so none can be folded into another)
 // and the element's last repetition writes its outputs.  A wave needs
 // ceil(sum rep / 64) rounds instead of max(rep), and the element fields are
-// read in order (coalesced) with one round trip to memory.
+// read in order (coalesced) with one round trip to memory.  NT: the
+// results are stored with the non-temporal hint (stream_put).
 // ---------------------------------------------------------------------------
 constexpr int kMapWords = 16; // rounds of 64 evaluations per start map
 constexpr int kMatSlots = 11; // LDS doubles per element: 7 EOS inputs, 4 outputs
 
-template <typename QG, bool RHO1, bool STRUCT>
+template <typename QG, bool RHO1, bool STRUCT, bool NT>
 __global__ __launch_bounds__(kBlock) void k_material_ordered(
     const ProbDesc *__restrict__ pds, const int32_t *__restrict__ blockProb,
     const uint16_t *__restrict__ erep, ElemFields f, const QG *__restrict__ delv_xi,
@@ -652,8 +654,8 @@ __global__ __launch_bounds__(kBlock) void k_material_ordered(
     if (P.eosvmin != 0.0 && vnewc < P.eosvmin) vnewc = P.eosvmin;
     if (P.eosvmax != 0.0 && vnewc > P.eosvmax) vnewc = P.eosvmax;
     if (valid) {
-        f.qq[i] = qquad;
-        f.ql[i] = qlin;
+        stream_put<NT>(&f.qq[i], qquad);
+        stream_put<NT>(&f.ql[i], qlin);
     }
     {
         double *me = mat_lds + kMatSlots * tid;
@@ -719,10 +721,10 @@ __global__ __launch_bounds__(kBlock) void k_material_ordered(
     if (valid) {
         const double *me = mat_lds + kMatSlots * tid;
         const double en = me[7], pn = me[8], qn = me[9], ssn = me[10];
-        f.ss[i] = ssn;
-        f.p[i] = pn;
-        f.e[i] = en;
-        f.q[i] = qn;
+        stream_put<NT>(&f.ss[i], ssn);
+        stream_put<NT>(&f.p[i], pn);
+        stream_put<NT>(&f.e[i], en);
+        stream_put<NT>(&f.q[i], qn);
         constraints(ssn, vdov, arealg, P, dtc, dth);
     }
     block_min_atomic(dtc, dth, minslots + size_t(prob) * kMinSlots * kSlotU64);
@@ -785,6 +787,12 @@ struct lulesh_ctx {
     int nprob = 0;
     int cus = 256;         // compute units of the device
     bool zSquare16 = true; // every problem's z-march patch is 16 x 16
+    // The kinematics and material kernels store their results with the
+    // non-temporal hint: contexts of one problem.  Measured at s = 60-250
+    // (FP64 and mixed): 1-3% less time per cycle, the next kernel's reads
+    // gaining more than the stores cost; the 64-problem batch on two
+    // streams ran 0.3-0.9% slower with it.
+    bool streamStores = false;
     std::vector<ProbHost> probs;
     int64_t totE = 0, totN = 0;
     Params P{};
@@ -982,6 +990,7 @@ int configure(lulesh_ctx *c) {
     std::vector<int32_t> zb, eb, mb, ob;
     c->hpd.clear();
     c->zSquare16 = true;
+    c->streamStores = c->nprob == 1;
     c->hsegs.clear();
     for (int pi = 0; pi < c->nprob; ++pi) {
         const ProbHost &ph = c->probs[size_t(pi)];
@@ -1098,12 +1107,22 @@ int enqueue_elements(lulesh_ctx *c) {
     const ElemKin ek{c->f[LULESH_F_VOLO], c->f[LULESH_F_V], c->f[LULESH_F_VNEW],
                      c->f[LULESH_F_DELV], c->f[LULESH_F_AREALG], c->f[LULESH_F_VDOV]};
     return launch(c, T_ELEMS, [&] {
-        auto kern = c->structured ? k_elems<R, true> : k_elems<R, false>;
+        auto kern = c->streamStores
+                        ? (c->structured ? k_elems<R, true, true> : k_elems<R, false, true>)
+                        : (c->structured ? k_elems<R, true, false> : k_elems<R, false, false>);
         kern<<<c->eBlocks, c->tune.block_elem, 0, c->stream>>>(
             c->d_pd, c->d_eBlockProb, c->nodelist, c->nb, ek, grad_ptr<R>(c, 0),
             grad_ptr<R>(c, 1), grad_ptr<R>(c, 2), grad_ptr<R>(c, 3), grad_ptr<R>(c, 4),
             grad_ptr<R>(c, 5), c->P.v_cut, c->P.eosvmin, c->P.eosvmax, c->st);
     });
+}
+
+template <typename R, bool NT>
+auto material_ordered_kernel(bool rho1, bool structured) {
+    return structured ? (rho1 ? k_material_ordered<R, true, true, NT>
+                              : k_material_ordered<R, false, true, NT>)
+                      : (rho1 ? k_material_ordered<R, true, false, NT>
+                              : k_material_ordered<R, false, false, NT>);
 }
 
 template <typename R>
@@ -1117,11 +1136,8 @@ int enqueue_material(lulesh_ctx *c) {
                             c->f[LULESH_F_SS]};
         return launch(c, T_MATERIAL, [&] {
             const bool rho1 = c->P.refdens == 1.0;
-            auto kern = c->structured
-                            ? (rho1 ? k_material_ordered<R, true, true>
-                                    : k_material_ordered<R, false, true>)
-                            : (rho1 ? k_material_ordered<R, true, false>
-                                    : k_material_ordered<R, false, false>);
+            auto kern = c->streamStores ? material_ordered_kernel<R, true>(rho1, c->structured)
+                                        : material_ordered_kernel<R, false>(rho1, c->structured);
             const size_t lds = size_t(c->tune.block_material) * (kMatSlots * sizeof(double) + 4) +
                                size_t(c->tune.block_material / 64) * kMapWords * 8;
             kern<<<c->oBlocks, c->tune.block_material, lds, c->stream>>>(
