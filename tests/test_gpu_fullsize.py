@@ -90,3 +90,49 @@ def test_s250_fp64_and_mixed(pyl):
     assert rep64[3] <= 1e-10   # FP64 symmetry: rounding order only
     assert repm[3] <= 1e-4     # FP32 hourglass / Q gradients
     devm.close()
+
+
+def first_values(pyl, dev, name, count):
+    """The first `count` values of a field (a partial download)."""
+    out = np.empty(count, np.float64)
+    L = pyl.lib()
+    assert L.lulesh_hip_download(dev.h, pyl.FIELD_ID[name], out.ctypes.data, count) == 0
+    return out
+
+
+def test_largest_mesh_first_motion(pyl):
+    """s = 645, the largest mesh the library accepts (E * 8 bytes just under
+    2^31: 268,336,125 elements; s = 646 is rejected, test_host_capi.py),
+    built in HBM.  Cycle 1 only turns e(0) into p(0) = (2/3) e(0) with every
+    node at rest; cycle 2's time step is 1.2 dt0 and it moves the eight
+    nodes of element 0 only: node (1,1,1) (mass h^3) along (1,1,1) with
+    p h^2/4 / h^3 * dt per axis, the symmetry-plane components pinned (the
+    closed forms of the oracle's first-motion test at s = 5)."""
+    s = 645
+    n = s + 1
+    h = 1.125 / s
+    banner(f"s={s}: first motion at the largest accepted size")
+    dev = pyl.Device.sedov(s, 11)
+    print(f"[fullsize] device bytes {dev.device_bytes() / 2 ** 30:.1f} GiB", flush=True)
+    ts = dev.time_state()
+    dt0 = ts.deltatime
+    einit = first_values(pyl, dev, "e", 1)[0]
+    assert dev.run(1, ts) == 0 and ts.cycle == 1
+    K2 = 2 * n * n  # node planes K = 0 and 1
+    for f in ("xd", "yd", "zd"):
+        assert np.all(first_values(pyl, dev, f, K2) == 0), f
+    p0 = first_values(pyl, dev, "p", 1)[0]
+    assert p0 == pytest.approx(2 / 3 * einit, rel=1e-15)
+    assert dev.run(2, ts) == 0 and ts.cycle == 2
+    assert ts.deltatime == pytest.approx(1.2 * dt0, rel=1e-15)
+    v = p0 * h * h / 4 / h ** 3 * ts.deltatime
+    vel = {f: first_values(pyl, dev, f, K2).reshape(2, n, n) for f in ("xd", "yd", "zd")}
+    for f in ("xd", "yd", "zd"):
+        assert vel[f][1, 1, 1] == pytest.approx(v, rel=1e-13), f
+    assert vel["xd"][0, 0, 1] > 0 and vel["yd"][0, 0, 1] == 0 and vel["zd"][0, 0, 1] == 0
+    moving = np.zeros((2, n, n), bool)
+    moving[:, :2, :2] = True
+    for f in ("xd", "yd", "zd"):
+        assert np.all(vel[f][~moving] == 0), f
+    assert np.all(vel["zd"][0] == 0)  # the z = 0 plane keeps zero z velocity
+    dev.close()

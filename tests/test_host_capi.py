@@ -105,6 +105,7 @@ def test_create_rejects_bad_views_without_device(pyl):
     # device-side set-up: argument checks come before any device call
     for args in [(0, 11, 1, 1, 0), (10, 0, 1, 1, 0), (10, 11, -1, 1, 0), (10, 11, 1, -1, 0),
                  (10, 11, 1, 1, pyl.OPT_GENERIC), (10, 11, 1, 1, 7), (700, 11, 1, 1, 0),
+                 (646, 11, 1, 1, 0),
                  (10, 21, 1, 100000, 0)]:
         assert L.lulesh_hip_create_sedov(*args, ctypes.byref(h)) == pyl.INVALID_ARGUMENT, args
     assert L.lulesh_hip_create_sedov(10, 11, 1, 1, 0, None) == pyl.INVALID_ARGUMENT
