@@ -48,6 +48,39 @@ def test_batch_equals_individual_runs(pyl):
                                           err_msg=f"problem {k} {name}")
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_ragged_batch_with_tiny_problems(pyl, mode):
+    """A batch mixing one- to eight-element problems (most regions empty)
+    with larger ones, run until every problem reaches its stop time: each
+    problem stops at its own cycle, and every problem's state equals the
+    same problem run alone, bit for bit, in both material kernels."""
+    banner(f"ragged batch, material_mode={mode}")
+    params = [(1, 11, 1.0), (14, 16, 0.8), (2, 11, 1.5), (1, 1, 0.7), (9, 21, 1.2),
+              (3, 21, 1.0)]
+    batch = pyl.Batch([pyl.make_domain(s, nr, f) for s, nr, f in params])
+    t = batch.tuning()
+    t.material_mode = mode
+    batch.set_tuning(t)
+    rc, states = batch.run(10 ** 7)
+    assert rc == 0
+    cycles = set()
+    for k, (s, nr, f) in enumerate(params):
+        dom = pyl.make_domain(s, nr, f)
+        dev = pyl.Device(dom)
+        dt = dev.tuning()
+        dt.material_mode = mode
+        dev.set_tuning(dt)
+        ts = dom.time_state()
+        assert dev.run(10 ** 7, ts) == 0
+        assert states[k].cycle == ts.cycle and states[k].time == ts.time, k
+        cycles.add(ts.cycle)
+        for name in FIELDS:
+            np.testing.assert_array_equal(batch.download(k, name), dev.download(name),
+                                          err_msg=f"problem {k} {name}")
+        dev.close()
+    assert len(cycles) > 1  # the problems stop at different cycles
+
+
 def test_batch_problem_matches_oracle(pyl):
     banner("batch member vs oracle (scaled energy)")
     doms = [pyl.make_domain(6, 11, 1.0), pyl.make_domain(10, 21, 1.8)]

@@ -138,6 +138,35 @@ def test_device_loop_vs_oracle(pyl, s, its, path):
     assert rep[1] <= max(orep[1] * 10, 1e-9 * abs(orep[0]))
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("path", ["structured", "generic"])
+@pytest.mark.parametrize("s,nr", [(1, 1), (1, 11), (2, 11), (3, 21), (4, 11)])
+def test_tiny_meshes_with_empty_regions(pyl, s, nr, path, mode):
+    """Meshes of 1 to 64 elements, most of whose regions are empty (s=1
+    with 11 regions: ten empty; s=3 with 21: nineteen), run to their stop
+    time in both material kernels (element order; region lists, which walk
+    the empty regions): the cycle count, the time and every state field as
+    the oracle's."""
+    banner(f"tiny s={s} nr={nr} {path} material_mode={mode}")
+    ora = OracleDomain(s, nr)
+    assert ora.run(10 ** 7) == 0
+    dom = pyl.Domain(s, nr)
+    assert sum(dom.view.regElemSize[r] for r in range(nr)) == s ** 3
+    dev = pyl.Device(dom, pyl.FP64, PATHS[path])
+    t = dev.tuning()
+    t.material_mode = mode
+    dev.set_tuning(t)
+    assert dev.tuning().material_mode == mode
+    ts = dom.time_state()
+    assert dev.run(10 ** 7, ts) == 0
+    dev.download_view()
+    assert ts.cycle == ora.cycle
+    assert abs(ts.time - ora.scalar("time")) <= 1e-12 * ora.scalar("time")
+    for f in STATE:
+        assert max_rel(dom.field(f), ora.f64(f)) < 1e-9, f
+    dev.close()
+
+
 @pytest.mark.parametrize("s,nr,b,c,its", [(12, 16, 2, 2, 150), (9, 1, 1, 1, 120),
                                            (14, 21, 3, 0, 150)])
 def test_device_loop_vs_oracle_region_options(pyl, s, nr, b, c, its):
