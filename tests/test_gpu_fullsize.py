@@ -136,3 +136,35 @@ def test_largest_mesh_first_motion(pyl):
         assert np.all(vel[f][~moving] == 0), f
     assert np.all(vel["zd"][0] == 0)  # the z = 0 plane keeps zero z velocity
     dev.close()
+
+
+def test_s150_to_stop_time_in_every_form(pyl):
+    """s = 150, 11 regions, built in HBM and run to its stop time (t = 0.01)
+    in the separately rounded and fused FP64 forms and both mixed forms:
+    every form stops at the same cycle; the fused origin energy is within
+    1e-9 of the separately rounded one and the mixed ones within 1e-6; the
+    x/y/z permutation symmetry of plane 0 holds to rounding in FP64
+    (MaxRelDiff <= 1e-10) and to FP32 level in the mixed forms."""
+    banner("s=150 to the stop time, four forms")
+    out = {}
+    for name, prec, opts in (("exact", pyl.FP64, 0), ("fused", pyl.FP64, pyl.OPT_FMA),
+                             ("mixed", pyl.MIXED, 0), ("mixed_fused", pyl.MIXED, pyl.OPT_FMA)):
+        dev = pyl.Device.sedov(150, 11, precision=prec, opts=opts)
+        ts = dev.time_state()
+        assert dev.run(10 ** 7, ts) == 0
+        rep = dev.report()
+        out[name] = (ts.cycle, ts.time, rep)
+        print(f"[fullsize] {name}: {ts.cycle} cycles, t={ts.time!r}, e0={rep[0]!r}, "
+              f"MaxRelDiff {rep[3]:.3e}", flush=True)
+        dev.close()
+    cycles = {c for c, _, _ in out.values()}
+    assert len(cycles) == 1
+    for c, t, _ in out.values():
+        assert t == pytest.approx(1.0e-2, rel=1e-12)
+    e = out["exact"][2][0]
+    assert abs(out["fused"][2][0] - e) <= 1e-9 * abs(e)
+    for name in ("mixed", "mixed_fused"):
+        assert abs(out[name][2][0] - e) <= 1e-6 * abs(e), name
+        assert out[name][2][3] <= 1e-3, name
+    for name in ("exact", "fused"):
+        assert out[name][2][3] <= 1e-10, name
