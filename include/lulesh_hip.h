@@ -188,7 +188,8 @@ typedef struct lulesh_tuning {
     int32_t block_elem;       /* threads per block of the kinematics kernel: 64, 128, 256
                                  (other values: 128) */
     int32_t block_material;   /* threads per block of the material/EOS kernel: 64, 128, 256
-                                 (other values: 128) */
+                                 (other values: 64 for a context of one problem, 128
+                                 for a batch) */
     int32_t cycles_per_graph; /* cycles captured per HIP graph (0 = by problem size) */
     int32_t material_mode;    /* 0: element order, EOS repetitions balanced over each wave;
                                  1: region lists grouped by repetition count */
