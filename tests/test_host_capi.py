@@ -37,6 +37,22 @@ def test_library_exports_every_declared_symbol(pyl):
     assert not missing, missing
 
 
+def test_headers_are_plain_c(tmp_path):
+    """The boundary headers compile as strict C99 (plain pointers, sizes and
+    structs: a C caller or an FFI generator can consume them) and as C++."""
+    src = tmp_path / "abi.c"
+    src.write_text('#include "lulesh_hip.h"\n#include "lulesh_domain.h"\n'
+                   "int main(void) {\n"
+                   "  return sizeof(lulesh_tuning) + sizeof(lulesh_time_state) +\n"
+                   "         sizeof(lulesh_domain_view) == 0;\n}\n")
+    inc = os.path.join(ROOT, "include")
+    for cmd in (["gcc", "-std=c99", "-pedantic", "-Wall", "-Wextra", "-Werror"],
+                ["g++", "-std=c++17", "-x", "c++", "-pedantic", "-Wall", "-Wextra", "-Werror"]):
+        r = subprocess.run(cmd + ["-I", inc, "-c", str(src), "-o", str(tmp_path / "abi.o")],
+                           capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0, (cmd[0], r.stderr)
+
+
 @pytest.mark.parametrize("s,nr,b,c", [(1, 11, 1, 1), (2, 11, 1, 1), (5, 11, 1, 1), (9, 16, 1, 2),
                                       (12, 21, 1, 1), (7, 5, 2, 0), (6, 1, 1, 1), (30, 11, 1, 1)])
 def test_host_domain_matches_oracle(pyl, s, nr, b, c):
