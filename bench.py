@@ -181,47 +181,27 @@ def copy_calibration(nbytes, reps):
                       "note": "per copy dispatch: nbytes read, nbytes written"}))
 
 
-def copy_bandwidth(nbytes=1 << 31, reps=10):
-    """Bytes moved per second (read + write) by the HIP runtime's
-    device-to-device copy of `nbytes` (hipMemcpyAsync, its blit kernel),
-    timed with HIP events: a practical HBM ceiling for kernels that stream
-    reads and writes, reported beside the 8 TB/s peak."""
-    import ctypes
+def copy_bandwidth(pyl, nbytes=1 << 31, reps=10):
+    """Bytes moved per second (read + write) by the library's hand-written
+    streaming copy of `nbytes` (16-byte non-temporal loads and stores, the
+    best of five launch shapes, HIP events): the practical HBM ceiling of a
+    kernel that streams reads and writes, reported beside the 8 TB/s peak."""
+    gbs = ctypes.c_double()
+    rc = pyl.lib().lulesh_hip_stream_bandwidth(ctypes.c_int64(nbytes), reps, ctypes.byref(gbs))
+    return gbs.value if rc == 0 and gbs.value > 0 else None
 
-    hip = None
-    for name in ("libamdhip64.so.7", "libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
-        try:
-            hip = ctypes.CDLL(name)  # already loaded by the library: the same handle
-            break
-        except OSError:
-            continue
-    if hip is None:
-        return None
-    vp = ctypes.c_void_p
-    src, dst, e0, e1 = vp(), vp(), vp(), vp()
-    if hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) or \
-            hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)):
-        return None
-    hip.hipMemset(src, 0, ctypes.c_size_t(nbytes))
-    hip.hipEventCreate(ctypes.byref(e0))
-    hip.hipEventCreate(ctypes.byref(e1))
-    D2D = 3  # hipMemcpyDeviceToDevice
-    for _ in range(3):
-        hip.hipMemcpyAsync(dst, src, ctypes.c_size_t(nbytes), D2D, None)
-    hip.hipEventRecord(e0, None)
-    for _ in range(reps):
-        hip.hipMemcpyAsync(dst, src, ctypes.c_size_t(nbytes), D2D, None)
-    hip.hipEventRecord(e1, None)
-    hip.hipEventSynchronize(e1)
-    ms = ctypes.c_float()
-    hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
-    for e in (e0, e1):
-        hip.hipEventDestroy(e)
-    hip.hipFree(src)
-    hip.hipFree(dst)
-    if ms.value <= 0:
-        return None
-    return 2 * nbytes * reps / (ms.value * 1e-3) / 1e9
+
+def measured_traffic_per_cycle(arith):
+    """HBM bytes one s=150 FP64 cycle moves by the counters of the committed
+    rocprofv3 summary (FETCH_SIZE + WRITE_SIZE of each kernel, calibrated),
+    or None."""
+    tot = 0.0
+    for k in ("nodal", "elements", "material"):
+        t = measured("traffic", k, arith)
+        if t is None:
+            return None
+        tot += t
+    return tot
 
 
 def main():
@@ -274,7 +254,10 @@ def main():
     dom = pyl.Domain(s, nr)
     t_up = time.perf_counter()
     AO = pyl.OPT_FMA if args.arith == "fma" else 0
-    dev = pyl.Device(dom, pyl.FP64, AO)  # allocation, upload of every field, layout checks
+    # allocation, upload of every field, layout checks; fx.. / xdd.. are not
+    # stored (dead between cycles, nothing reads them)
+    NF = pyl.OPT_NO_NODE_FORCES
+    dev = pyl.Device(dom, pyl.FP64, AO | NF)
     t_up = time.perf_counter() - t_up
     elapsed, ts, tuning = timed_run(pyl, dev, dom, args.warmup, args.steps, args.tune_cycles)
     dev_ms = dev.last_run_ms()  # HIP events around the replayed graphs of the timed region
@@ -331,7 +314,7 @@ def main():
     if not args.no_other_form:
         oname = "exact" if args.arith == "fma" else "fma"
         domo = pyl.Domain(s, nr)
-        devo = pyl.Device(domo, pyl.FP64, pyl.OPT_FMA if oname == "fma" else 0)
+        devo = pyl.Device(domo, pyl.FP64, (pyl.OPT_FMA if oname == "fma" else 0) | NF)
         el_o, tso, _ = timed_run(pyl, devo, domo, args.warmup, args.steps, args.tune_cycles)
         dev_ms_o = devo.last_run_ms()
         devo.set_timing(True)
@@ -359,7 +342,7 @@ def main():
     mixed = None
     if not args.no_mixed:
         domm = pyl.Domain(s, nr)
-        devm = pyl.Device(domm, pyl.MIXED, AO)
+        devm = pyl.Device(domm, pyl.MIXED, AO | NF)
         el_m, tsm, _ = timed_run(pyl, devm, domm, args.warmup, args.steps, args.tune_cycles)
         rc = devm.run(tsm.cycle + PROF_CYCLES, tsm)  # to the cycle the FP64 state is read at
         assert rc == 0, rc
@@ -410,14 +393,37 @@ def main():
         batch = batch_bench(pyl, args.batch, args.batch_its, 0, args.batch_tune_cycles, pyl.FP64,
                             args.batch_streams, AO)
         print(f"[bench] batch: {batch['value']:.4e} zc/s", file=sys.stderr, flush=True)
+        # the single s=150 problem over as many cycles as the batch runs, in
+        # the same form, for the batch-vs-single comparison
+        dl = pyl.Device.sedov(s, nr, precision=pyl.FP64, opts=AO)
+        tl = dl.time_state()
+        assert dl.run(args.warmup, tl) == 0
+        c0 = tl.cycle
+        t0 = time.perf_counter()
+        assert dl.run(c0 + args.batch_its, tl) == 0
+        el = time.perf_counter() - t0
+        single = {"value": E * (tl.cycle - c0) / el, "cycles": tl.cycle - c0,
+                  "device_ms_per_step": dl.last_run_ms() / (tl.cycle - c0)}
+        dl.close()
+        batch["single_s150_same_cycles"] = single
+        batch["vs_single_s150"] = batch["value"] / single["value"]
+        print(f"[bench] single s={s} over {single['cycles']} cycles: {single['value']:.4e} zc/s",
+              file=sys.stderr, flush=True)
 
     # ---- CPU baseline ----------------------------------------------------
     cpu = None
     if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
+        # every host core this process may run on, unless the environment
+        # sets an OpenMP thread budget below that (the GPU box allots 16
+        # threads per GPU); both figures go into the line
+        avail = len(os.sched_getaffinity(0))
+        budget = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        threads = max(1, min(avail, budget) if budget > 0 else avail)
         cs = min(s, 150)
         v, ncyc, el = cpu_baseline(cs, nr, args.cpu_seconds, threads)
         cpu = {"value": v, "unit": "zone-cycles/s", "cores": threads, "kind": "port",
+               "host_cpus_available": avail, "host_cpus_total": os.cpu_count(),
+               "omp_num_threads_env": budget or None,
                "sample": f"OpenMP oracle, s={cs}, {nr} regions, {ncyc} cycles after cycle 1, "
                          f"{el:.1f}s wall, gcc -O3 -ffp-contract=off"}
         # the same port on one core, a shorter sample
@@ -426,8 +432,9 @@ def main():
                          "sample": f"serial oracle, s={cs}, {nr} regions, {ncyc1} cycles after "
                                    f"cycle 1, {el1:.1f}s wall"}
 
-    copy_gbs = None if args.no_copy_reference else copy_bandwidth()
+    copy_gbs = None if args.no_copy_reference else copy_bandwidth(pyl)
     bsz = model_bytes_per_zone_cycle(s)
+    traffic_cycle = measured_traffic_per_cycle(args.arith) if prof else None
     out = {
         "metric": "zone-cycles/sec FP64 s=150 vs FP32-mixed & 64-problem batch; % HBM peak",
         "value": zcps,
@@ -457,9 +464,14 @@ def main():
                      "fp64_vector": fp64,
                      "source_of_traffic_and_valu_busy": TRAFFIC_FILES[args.arith]
                      if traffic else None},
-        "hbm_copy_reference_GBs": round(copy_gbs, 1) if copy_gbs else None,
+        "hbm_stream_copy_GBs": round(copy_gbs, 1) if copy_gbs else None,
         "model_bytes_per_zone_cycle": round(bsz, 1),
         "model_hbm_frac": round(bsz * zcps / (HBM_PEAK_GBS * 1e9), 4),
+        # bytes the kernels actually move per cycle (rocprofv3 counters of the
+        # committed summary) over the measured time per cycle
+        "measured_bytes_per_cycle": int(traffic_cycle) if traffic_cycle else None,
+        "measured_hbm_frac": round(traffic_cycle / (elapsed / cycles) / (HBM_PEAK_GBS * 1e9), 4)
+        if traffic_cycle else None,
         "kernels": kernels,
         "ms_per_cycle_kernel_sum": round(ms_cycle_events, 4),
         "value_incl_host_transfers": E * cycles / (elapsed + t_up + t_down),

@@ -51,6 +51,17 @@ int32_t lulesh_region_rep(int32_t reg, int32_t numReg, int32_t cost);
  * deposit (lulesh-init.cc:172-181).  For ensembles of Sedov problems. */
 int lulesh_domain_scale_energy(lulesh_domain *d, double factor);
 
+/* The visualization dump of DumpToVisit (lulesh-viz.cc:27-197, called for
+ * -v at lulesh.cc:2397-2400) from the Domain's current host arrays, as VTK
+ * legacy unstructured-grid files (VisIt reads them; the reference's Silo
+ * library is not used): mesh, regions, e/p/v/q per element and
+ * speed/xd/yd/zd per node, time and cycle.  Writes
+ * <dir>/lulesh_plot_c<cycle>.vtk, or for pieces > 1 (-f) that many slabs of
+ * element planes, lulesh_plot_c<cycle>.<k>.vtk, listed in
+ * lulesh_plot_c<cycle>.visit.  0 on success, -3 bad arguments, -4 a file
+ * could not be written. */
+int lulesh_domain_write_viz(lulesh_domain *d, const char *dir, int32_t pieces);
+
 /* Parameters of a seeded ensemble (no counterpart in the reference): problem
  * sizes uniform in [smin, smax], region counts from {11, 16, 21}, energy
  * factors uniform in [0.5, 2), all drawn from one std::mt19937_64 stream

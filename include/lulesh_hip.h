@@ -45,8 +45,17 @@ extern "C" {
 #define LULESH_PRECISION_FP64 0  /* every kernel in FP64 (the reference's Real_t, lulesh.h:27) */
 #define LULESH_PRECISION_MIXED 1 /* hourglass + monotonic-Q gradients in FP32, the rest FP64 */
 /* Option bits or-ed into the precision argument of lulesh_hip_create. */
-#define LULESH_OPT_NODE_FORCES 0x100 /* keep fx/fy/fz and xdd/ydd/zdd up to date in HBM
-                                        (they are dead between cycles; off by default) */
+#define LULESH_OPT_NODE_FORCES 0x100 /* keep fx/fy/fz and xdd/ydd/zdd up to date in HBM, as
+                                        the reference's Domain does (lulesh.cc:858-860,
+                                        1115-1123).  The default of lulesh_hip_create (a
+                                        host Domain's view); batches and problems built in
+                                        HBM keep them only with this bit. */
+#define LULESH_OPT_NO_NODE_FORCES 0x800 /* lulesh_hip_create only: do not write fx.. and
+                                        xdd.. (dead between cycles: nothing reads them
+                                        before the next cycle overwrites them).  Their
+                                        downloads are then refused, and
+                                        lulesh_hip_download_view leaves those six host
+                                        arrays untouched. */
 #define LULESH_OPT_GENERIC 0x200     /* use the indexed kernels (nodelist + corner CSR)
                                         even when the mesh is the structured Sedov cube */
 #define LULESH_OPT_FMA 0x400        /* contract FP64 (and FP32) multiply-adds into fused
@@ -160,7 +169,11 @@ int lulesh_hip_constraints_result(lulesh_ctx *ctx, double *dtcourant, double *dt
 int lulesh_hip_download(lulesh_ctx *ctx, int field_id, void *dst, size_t count);
 int lulesh_hip_upload(lulesh_ctx *ctx, int field_id, const void *src, size_t count);
 /* Download every state field into a view (NULL pointers are skipped), and
- * the time state. */
+ * the time state.  fx/fy/fz and xdd/ydd/zdd are skipped (their host arrays
+ * left as they are) when the context does not keep them
+ * (LULESH_OPT_NO_NODE_FORCES, or a context built without
+ * LULESH_OPT_NODE_FORCES by the batch or in-HBM set-up calls); a single
+ * lulesh_hip_download of one of them then returns LULESH_INVALID_ARGUMENT. */
 int lulesh_hip_download_view(lulesh_ctx *ctx, lulesh_domain_view *dst);
 
 /* Kernel timing of the last lulesh_hip_run call that had timing enabled:
@@ -179,6 +192,13 @@ int32_t lulesh_hip_is_structured(const lulesh_ctx *ctx);
  * (HIP events around the replayed graphs, polling included; for a stream
  * group the slowest stream's). */
 double lulesh_hip_last_run_ms(const lulesh_ctx *ctx);
+
+/* HBM ceiling for the roofline: the best rate (bytes read + written per
+ * second, in GB/s) of a hand-written streaming copy of `bytes` bytes
+ * (16-byte vectors, non-temporal loads and stores, five launch shapes, each
+ * timed over `reps` copies with HIP events).  Allocates and frees its own
+ * two buffers.  A diagnostic with no counterpart in the reference. */
+int lulesh_hip_stream_bandwidth(int64_t bytes, int32_t reps, double *GBs);
 
 /* ---------------------------------------------------------------------- */
 /* Launch tuning.  No counterpart in the reference, whose knob is the HPX

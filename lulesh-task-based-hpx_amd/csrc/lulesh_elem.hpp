@@ -26,15 +26,19 @@ enum : int32_t {
 // Device-resident time state and flags of one problem (the Domain's time
 // members, lulesh.h:563-572, plus the loop condition of lulesh.cc:2374-2375).
 // `cur` selects the half of the node double buffer that holds the state at
-// the start of the cycle; the nodal phase writes the other half, and the
-// next cycle adopts it only if this cycle ran (`active`), so cycles skipped
-// after the stop condition leave the buffers consistent.
+// the start of the cycle; the nodal phase writes the other half (`fresh`),
+// and the next cycle adopts it only if a nodal phase wrote it, so cycles
+// skipped after the stop condition, and phases called on their own, leave
+// the buffers consistent.
 struct DevState {
     double time, deltatime, stoptime, dtfixed, deltatimemultlb, deltatimemultub, dtmax;
     unsigned long long dtc_bits, dth_bits; // constraint minima (bit patterns of positive doubles)
     int32_t cycle, its, active, error;
-    int32_t cur;
+    int32_t cur;     // half of the node double buffer that holds the cycle's start state
     int32_t pending; // the last cycle's constraint minima are still in the slots
+    int32_t fresh;   // 1: the current cycle's nodal phase writes (or wrote) half cur ^ 1,
+                     // so the newest node state is in half cur ^ fresh
+    int32_t pad;
 };
 
 // Both halves of the node double buffer: b[h][0..5] = x, y, z, xd, yd, zd.

@@ -158,10 +158,11 @@ __global__ __launch_bounds__(kMinSlots) void k_time_increment(DevState *states,
         s.dtc_bits = st->dtc_bits = c;
         s.dth_bits = st->dth_bits = h;
     }
-    if (s.active) st->cur = s.cur ^ 1; // adopt the node state the last cycle wrote
+    if (s.fresh) st->cur = s.cur ^ 1; // adopt the node state the last cycle wrote
     const bool run = !s.error && (s.time < s.stoptime) && (s.cycle < s.its);
     st->active = run ? 1 : 0;
     st->pending = run ? 1 : 0;
+    st->fresh = run ? 1 : 0; // this cycle's nodal phase writes the other half
     if (!run) return;
     double targetdt = s.stoptime - s.time;
     double deltatime = s.deltatime;
@@ -194,10 +195,14 @@ __global__ __launch_bounds__(kMinSlots) void k_time_increment(DevState *states,
     st->dth_bits = dbits(1.0e+20);
 }
 
-// A cycle (or a nodal phase) driven by a host-side TimeIncrement: adopt the
-// previous cycle's node state if asked, take the host's dt, reset minima.
+// A cycle (or a nodal phase) driven by a host-side TimeIncrement: a nodal
+// phase (`adopt`) first adopts the node state the previous nodal phase
+// wrote; take the host's dt, reset minima.
 __global__ void k_begin_cycle(DevState *st, double deltatime, int reset_minima, int adopt) {
-    if (adopt && st->active) st->cur ^= 1;
+    if (adopt) {
+        if (st->fresh) st->cur ^= 1;
+        st->fresh = 1;
+    }
     st->deltatime = deltatime;
     st->active = 1;
     st->pending = reset_minima ? 1 : 0; // phases that produce minima
@@ -206,6 +211,29 @@ __global__ void k_begin_cycle(DevState *st, double deltatime, int reset_minima, 
         st->dtc_bits = dbits(1.0e+20);
         st->dth_bits = dbits(1.0e+20);
     }
+}
+
+// The time state and cycle limit of lulesh_hip_run, set on the device in
+// stream order (no host round trip); the node-buffer bookkeeping (cur,
+// fresh, active) stays as the device has it.  The caller's minima replace
+// any left in the slots.
+__global__ void k_set_time_state(DevState *st, double time, double deltatime,
+                                 unsigned long long dtc_bits, unsigned long long dth_bits,
+                                 int32_t cycle, int32_t its) {
+    st->time = time;
+    st->deltatime = deltatime;
+    st->dtc_bits = dtc_bits;
+    st->dth_bits = dth_bits;
+    st->pending = 0;
+    st->cycle = cycle;
+    st->its = its;
+    st->error = 0;
+}
+
+// The cycle limit of every problem of a batch.
+__global__ void k_set_its(DevState *st, int32_t nprob, int32_t its) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nprob) st[p].its = its;
 }
 
 __device__ __forceinline__ void load8(const double *__restrict__ f, const int n[8], double o[8]) {
@@ -353,7 +381,9 @@ __global__ __launch_bounds__(kBlock) void k_nodes(
 // volume check and UpdateVolumes) and CalcMonotonicQGradientsForElemsTask
 // (lulesh.cc:1277-1418).  Gradients are FP64, or FP32 in the mixed form.
 // STRUCT: node indices from the element's (k,j,i); else from nodelist.
-// The end-of-step node state is the half written by the nodal phase.
+// The end-of-step node state is the newest half (cur ^ fresh): the one the
+// nodal phase of this cycle wrote, or the start state when no nodal phase
+// ran since (a phase called on its own).
 // NT: the results are stored with the non-temporal hint (stream_put).
 // ---------------------------------------------------------------------------
 struct ElemKin {
@@ -376,7 +406,7 @@ __global__ __launch_bounds__(kBlock) void k_elems(
     if (li >= pd.E || !st->active) return;
     const int i = pd.elemOff + li; // element index in the concatenated arrays
     const double deltaTime = st->deltatime;
-    double *const *nf = nb.b[st->cur ^ 1];
+    double *const *nf = nb.b[st->cur ^ st->fresh];
     int n[8];
     if (STRUCT) {
         // element (k,j,i) -> its eight nodes, lulesh-init.cc:236-244
@@ -758,6 +788,32 @@ __global__ void k_narrow(int32_t n, const double *__restrict__ src, T *__restric
 
 inline unsigned grid_for(int64_t n, int block) { return unsigned((n + block - 1) / block); }
 
+// Streaming copy for the HBM ceiling measurement: every lane moves UNROLL
+// 16-byte vectors per pass, loads first, with the non-temporal hint on both
+// sides; GRID_STRIDE: a fixed grid walks the buffer, else one pass per lane.
+typedef double vec2d __attribute__((ext_vector_type(2)));
+
+template <int UNROLL, bool GRID_STRIDE>
+__global__ __launch_bounds__(256) void k_stream_copy(const vec2d *__restrict__ src,
+                                                     vec2d *__restrict__ dst, int64_t n) {
+    const int64_t lanes = int64_t(gridDim.x) * blockDim.x;
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    do {
+        vec2d v[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const int64_t k = i + int64_t(u) * lanes;
+            if (k < n) v[u] = __builtin_nontemporal_load(src + k);
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const int64_t k = i + int64_t(u) * lanes;
+            if (k < n) __builtin_nontemporal_store(v[u], dst + k);
+        }
+        i += int64_t(UNROLL) * lanes;
+    } while (GRID_STRIDE && i < n);
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------
@@ -828,8 +884,11 @@ struct lulesh_ctx {
     DevState *st = nullptr;
     DevState *hst = nullptr; // pinned mirror of all problems' states
     std::vector<DevState> init;
-    hipGraphExec_t graph = nullptr;
-    int graph_cycles = 0;
+    // captured cycle graphs: graphs[k] replays graph_sizes[k] cycles; the
+    // sizes are cycles_per_graph and the powers of two below it, so any
+    // count of cycles up to cycles_per_graph is a sum of distinct graphs
+    std::vector<hipGraphExec_t> graphs;
+    std::vector<int> graph_sizes;
     bool timing = false;
     std::vector<KernelTimer> timers;
     hipEvent_t run0 = nullptr, run1 = nullptr;
@@ -877,6 +936,7 @@ int upload(lulesh_ctx *c, T *dst, const T *src, size_t n) {
 bool is_node_field(int id) { return id <= LULESH_F_NODALMASS; }
 bool is_state_node_field(int id) { return id >= LULESH_F_X && id <= LULESH_F_ZD; }
 bool is_grad_field(int id) { return id >= LULESH_F_DELV_XI && id <= LULESH_F_DELX_ZETA; }
+bool is_force_field(int id) { return id >= LULESH_F_XDD && id <= LULESH_F_FZ; }
 
 double *view_field(const lulesh_domain_view *v, int id) {
     double *const tab[LULESH_F_COUNT] = {
@@ -977,6 +1037,13 @@ int auto_zplanes(const lulesh_ctx *c) {
     return std::min(4, nmax);
 }
 
+void destroy_graphs(lulesh_ctx *c) {
+    for (hipGraphExec_t g : c->graphs)
+        if (g) (void)hipGraphExecDestroy(g);
+    c->graphs.clear();
+    c->graph_sizes.clear();
+}
+
 // Rebuild the block -> problem / segment tables for the current tuning.
 int configure(lulesh_ctx *c) {
     lulesh_tuning &t = c->tune;
@@ -1051,10 +1118,7 @@ int configure(lulesh_ctx *c) {
         (rc = put(&c->d_zBlockProb, zb)) || (rc = put(&c->d_eBlockProb, eb)) ||
         (rc = put(&c->d_mBlockSeg, mb)) || (rc = put(&c->d_oBlockProb, ob)))
         return rc;
-    if (c->graph) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
-    }
+    destroy_graphs(c);
     return 0;
 }
 
@@ -1200,13 +1264,9 @@ int enqueue_cycle(lulesh_ctx *c) {
     return enqueue_cycle_body(c);
 }
 
-int build_graph(lulesh_ctx *c) {
-    const int cycles = c->tune.cycles_per_graph;
-    if (c->graph && c->graph_cycles == cycles) return 0;
-    if (c->graph) {
-        (void)hipGraphExecDestroy(c->graph);
-        c->graph = nullptr;
-    }
+// Capture `cycles` cycles into one executable graph, uploaded to the device
+// so that its first replay costs no more than the later ones.
+int capture_cycles(lulesh_ctx *c, int cycles, hipGraphExec_t *out) {
     hipGraph_t g = nullptr;
     HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = 0;
@@ -1217,10 +1277,47 @@ int build_graph(lulesh_ctx *c) {
         return rc;
     }
     if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
-    e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return hip_fail(c, e, "hipGraphInstantiate");
-    c->graph_cycles = cycles;
+    HIP_TRY(c, hipGraphUpload(*out, c->stream));
+    return 0;
+}
+
+// The graph set of the current tuning: cycles_per_graph cycles, and every
+// power of two below it (built together, before any timed replay).
+int build_graphs(lulesh_ctx *c) {
+    const int per = std::max(c->tune.cycles_per_graph, 1);
+    if (!c->graph_sizes.empty() && c->graph_sizes[0] == per) return 0;
+    destroy_graphs(c);
+    std::vector<int> sizes{per};
+    for (int p = 1; p < per; p <<= 1) sizes.push_back(p);
+    std::sort(sizes.begin() + 1, sizes.end(), std::greater<int>());
+    for (int n : sizes) {
+        hipGraphExec_t g = nullptr;
+        if (int rc = capture_cycles(c, n, &g)) {
+            destroy_graphs(c);
+            return rc;
+        }
+        c->graphs.push_back(g);
+        c->graph_sizes.push_back(n);
+    }
+    return 0;
+}
+
+// Enqueue exactly m <= cycles_per_graph cycles: the full graph, or a sum of
+// distinct power-of-two graphs (timing mode: the launches themselves).
+int enqueue_cycles(lulesh_ctx *c, int m) {
+    if (c->timing) {
+        for (int k = 0; k < m; ++k)
+            if (int rc = enqueue_cycle(c)) return rc;
+        return 0;
+    }
+    for (size_t k = 0; k < c->graphs.size() && m > 0; ++k)
+        if (c->graph_sizes[k] <= m) {
+            HIP_TRY(c, hipGraphLaunch(c->graphs[k], c->stream));
+            m -= c->graph_sizes[k];
+        }
     return 0;
 }
 
@@ -1242,23 +1339,59 @@ bool still_running(const DevState &h) {
     return !h.error && h.time < h.stoptime && h.cycle < h.its;
 }
 
-// Replay cycle graphs (or time every launch) until no problem runs.
+// Cycles (up to cap) that a problem in host state h certainly still runs:
+// the loop condition of lulesh.cc:2374-2375 holds before each of them.
+// TimeIncrement (lulesh.cc:182-229) never grows the time step by more than
+// deltatimemultub per cycle, so the time before the j-th next cycle is at
+// most time + dt (u + u^2 + ... + u^j); the bound is padded against
+// rounding.  It only decides how far the host may queue cycles ahead of its
+// next look at the device; the device evaluates the condition itself.
+int64_t sure_cycles(const DevState &h, int64_t cap) {
+    if (!still_running(h)) return 0;
+    const double u = std::max(h.deltatimemultub, 1.0) * (1.0 + 1e-12);
+    double t = h.time, dt = h.deltatime;
+    int64_t j = 1;
+    while (j < cap && int64_t(h.cycle) + j < int64_t(h.its)) {
+        dt *= u;
+        t += dt * (1.0 + 1e-12);
+        if (!(t < h.stoptime)) break;
+        ++j;
+    }
+    return j;
+}
+
+// Advance every problem until none runs.  The host queues cycles while it
+// can tell from the last state it read that the first queued cycle will run
+// for some problem (a problem's cycle limit is exact, its stop time is
+// bounded by sure_cycles), at most cycles_per_graph at a time and never past
+// the largest cycle limit; only when that is uncertain does it wait for the
+// device and read the states again.  A run of k cycles towards a cycle
+// limit therefore replays exactly k cycles with no wait in between; near the
+// stop time it reads the state after every replay.
 int run_loop(lulesh_ctx *c) {
     int rc = 0;
-    if (!c->timing && (rc = build_graph(c))) return rc;
-    const int per = c->tune.cycles_per_graph;
+    if (!c->timing && (rc = build_graphs(c))) return rc;
+    const int per = std::max(c->tune.cycles_per_graph, 1);
     HIP_TRY(c, hipEventRecord(c->run0, c->stream));
+    int64_t queued = 0; // cycles enqueued since the host states were read
     for (;;) {
-        if (c->timing) {
-            for (int k = 0; k < per && rc == 0; ++k) rc = enqueue_cycle(c);
-            if (rc) return rc;
-        } else {
-            HIP_TRY(c, hipGraphLaunch(c->graph, c->stream));
+        int64_t sure = 0, maybe = 0;
+        for (int pi = 0; pi < c->nprob; ++pi) {
+            const DevState &h = c->hst[pi];
+            if (!still_running(h)) continue;
+            sure = std::max(sure, sure_cycles(h, queued + per));
+            maybe = std::max(maybe, int64_t(h.its) - int64_t(h.cycle));
+        }
+        if (sure > queued) {
+            const int m = int(std::min<int64_t>(per, maybe - queued));
+            if ((rc = enqueue_cycles(c, m))) return rc;
+            queued += m;
+            if (!c->timing) continue;
+        } else if (queued == 0) {
+            break; // the states are current and no problem runs
         }
         if ((rc = read_states(c))) return rc;
-        bool any = false;
-        for (int pi = 0; pi < c->nprob; ++pi) any = any || still_running(c->hst[pi]);
-        if (!any) break;
+        queued = 0;
     }
     // the last cycle's minima, for the time state handed back to the caller
     if ((rc = enqueue_minima(c))) return rc;
@@ -1352,6 +1485,26 @@ bool view_ok(const lulesh_domain_view *v) {
            int64_t(v->numElem) * 8 < (int64_t(1) << 31);
 }
 
+// Every element in exactly one region list (lulesh-init.cc:341-445 builds
+// a partition): the material kernels take each element's repetition count
+// from its region, and the element-order kernel maps the EOS evaluations of
+// a wave onto its elements by those counts.
+bool regions_ok(const lulesh_domain_view *v) {
+    std::vector<uint8_t> hit(size_t(v->numElem), 0);
+    int64_t seen = 0;
+    for (int r = 0; r < v->numReg; ++r) {
+        const int32_t n = v->regElemSize[r];
+        if (n < 0 || (n > 0 && !v->regElemlist[r])) return false;
+        for (int32_t k = 0; k < n; ++k) {
+            const int32_t el = v->regElemlist[r][k];
+            if (el < 0 || el >= v->numElem || hit[size_t(el)]) return false;
+            hit[size_t(el)] = 1;
+        }
+        seen += n;
+    }
+    return seen == v->numElem;
+}
+
 Params params_of(const lulesh_domain_view *v) {
     return Params{v->e_cut, v->p_cut, v->q_cut, v->v_cut, v->u_cut, v->hgcoef,
                   v->monoq_max_slope, v->monoq_limiter_mult, v->qlc_monoq, v->qqc_monoq, v->qqc,
@@ -1374,18 +1527,24 @@ DevState state_of(const lulesh_domain_view *v) {
     return s;
 }
 
+// Views of host Domains into a context.  single_view: lulesh_hip_create,
+// whose context keeps fx.. and xdd.. unless LULESH_OPT_NO_NODE_FORCES.
 int create_impl(const lulesh_domain_view *const *views, int nprob, int precision_opts,
-                lulesh_ctx **out) {
+                bool single_view, lulesh_ctx **out) {
     if (!out) return LULESH_INVALID_ARGUMENT;
     *out = nullptr;
     const int precision = precision_opts & 0xff;
-    if ((precision_opts & ~(0xff | LULESH_OPT_NODE_FORCES | LULESH_OPT_GENERIC)) ||
+    const int forces_opts = precision_opts & (LULESH_OPT_NODE_FORCES | LULESH_OPT_NO_NODE_FORCES);
+    if ((precision_opts & ~(0xff | LULESH_OPT_NODE_FORCES | LULESH_OPT_NO_NODE_FORCES |
+                            LULESH_OPT_GENERIC)) ||
+        forces_opts == (LULESH_OPT_NODE_FORCES | LULESH_OPT_NO_NODE_FORCES) ||
+        (!single_view && (forces_opts & LULESH_OPT_NO_NODE_FORCES)) ||
         (precision != LULESH_PRECISION_FP64 && precision != LULESH_PRECISION_MIXED) ||
         nprob < 1 || !views)
         return LULESH_INVALID_ARGUMENT;
     int64_t totE = 0, totN = 0;
     for (int pi = 0; pi < nprob; ++pi) {
-        if (!view_ok(views[pi])) return LULESH_INVALID_ARGUMENT;
+        if (!view_ok(views[pi]) || !regions_ok(views[pi])) return LULESH_INVALID_ARGUMENT;
         totE += views[pi]->numElem;
         totN += views[pi]->numNode;
         if (std::memcmp(&views[pi]->e_cut, &views[0]->e_cut, sizeof(double) * 19) != 0)
@@ -1403,7 +1562,9 @@ int create_impl(const lulesh_domain_view *const *views, int nprob, int precision
         return rc;
     };
     c->precision = precision;
-    c->store_forces = (precision_opts & LULESH_OPT_NODE_FORCES) != 0;
+    // (the indexed kernels always write them)
+    c->store_forces = !structured || (precision_opts & LULESH_OPT_NODE_FORCES) != 0 ||
+                      (single_view && !(precision_opts & LULESH_OPT_NO_NODE_FORCES));
     c->structured = structured;
     c->nprob = nprob;
     c->totE = totE;
@@ -1605,6 +1766,11 @@ int field_copy(lulesh_ctx *c, int prob, int id, void *host, size_t count, bool d
     const size_t n = is_node_field(id) ? size_t(ph.N) : size_t(ph.E);
     const size_t off = is_node_field(id) ? size_t(ph.nodeOff) : size_t(ph.elemOff);
     if (count > n) return LULESH_INVALID_ARGUMENT;
+    if (is_force_field(id) && !c->store_forces) {
+        c->err = "fx/fy/fz and xdd/ydd/zdd are not kept by this context "
+                 "(LULESH_OPT_NODE_FORCES was not given)";
+        return LULESH_INVALID_ARGUMENT;
+    }
     int rc = read_states(c);
     if (rc) return rc;
     if (c->precision == LULESH_PRECISION_MIXED && is_grad_field(id)) {
@@ -1632,7 +1798,7 @@ int field_copy(lulesh_ctx *c, int prob, int id, void *host, size_t count, bool d
     double *base = c->f[id];
     if (is_state_node_field(id)) {
         const DevState &h = c->hst[prob];
-        base = c->nb.b[h.cur ^ (h.active ? 1 : 0)][id - LULESH_F_X];
+        base = c->nb.b[h.cur ^ h.fresh][id - LULESH_F_X];
     }
     double *dev = base + off;
     HIP_TRY(c, hipMemcpyAsync(down ? host : static_cast<void *>(dev),
@@ -1962,14 +2128,14 @@ int make_group(const std::vector<std::vector<int32_t>> &parts, int nprob, Make &
 // ---------------------------------------------------------------------------
 int lulesh_hip_create(const lulesh_domain_view *v, int precision_opts, lulesh_ctx **out) {
     const lulesh_domain_view *views[1] = {v};
-    return create_impl(views, 1, precision_opts, out);
+    return create_impl(views, 1, precision_opts, true, out);
 }
 
 int lulesh_hip_batch_create(const lulesh_domain_view *const *views, int32_t nprob,
                             int precision_opts, lulesh_ctx **out) {
     const int k = requested_streams(precision_opts, nprob);
     const int opts = precision_opts & ~LULESH_OPT_STREAMS(0xf);
-    if (k == 1 || !views || !out) return create_impl(views, nprob, opts, out);
+    if (k == 1 || !views || !out) return create_impl(views, nprob, opts, false, out);
     std::vector<int64_t> E(static_cast<size_t>(nprob));
     for (int p = 0; p < nprob; ++p) {
         if (!views[p]) return LULESH_INVALID_ARGUMENT;
@@ -1979,7 +2145,7 @@ int lulesh_hip_batch_create(const lulesh_domain_view *const *views, int32_t npro
     return make_group(parts, nprob, [&](size_t l, lulesh_ctx **lane) {
         std::vector<const lulesh_domain_view *> sub;
         for (int32_t p : parts[l]) sub.push_back(views[p]);
-        return create_impl(sub.data(), int(sub.size()), opts, lane);
+        return create_impl(sub.data(), int(sub.size()), opts, false, lane);
     }, out);
 }
 
@@ -2090,19 +2256,22 @@ int lulesh_hip_constraints_result(lulesh_ctx *c, double *dtcourant, double *dthy
 
 int lulesh_hip_run(lulesh_ctx *c, int32_t max_cycles, lulesh_time_state *io) {
     if (!c || !io || c->nprob != 1) return LULESH_INVALID_ARGUMENT;
-    int rc = read_states(c); // keeps the node-buffer bookkeeping (cur, active)
-    if (rc) return rc;
+    // the host mirror the run loop plans with: the run constants of the
+    // problem and the caller's time state
     DevState &s = c->hst[0];
+    s = c->init[0];
     s.time = io->time;
     s.deltatime = io->deltatime;
     s.dtc_bits = host_dbits(io->dtcourant);
     s.dth_bits = host_dbits(io->dthydro);
-    s.pending = 0; // the caller's minima replace any left in the slots
+    s.pending = 0;
     s.cycle = io->cycle;
     s.its = max_cycles;
     s.error = 0;
-    if ((rc = write_states(c))) return rc;
-    rc = run_loop(c);
+    k_set_time_state<<<1, 1, 0, c->stream>>>(c->st, s.time, s.deltatime, s.dtc_bits,
+                                             s.dth_bits, s.cycle, s.its);
+    HIP_TRY(c, hipGetLastError());
+    const int rc = run_loop(c);
     to_time_state(c->hst[0], io);
     return rc;
 }
@@ -2137,7 +2306,8 @@ int lulesh_hip_batch_run(lulesh_ctx *c, int32_t max_cycles, lulesh_time_state *s
     int rc = read_states(c);
     if (rc) return rc;
     for (int pi = 0; pi < c->nprob; ++pi) c->hst[pi].its = max_cycles;
-    if ((rc = write_states(c))) return rc;
+    k_set_its<<<grid_for(c->nprob, 64), 64, 0, c->stream>>>(c->st, c->nprob, max_cycles);
+    HIP_TRY(c, hipGetLastError());
     rc = run_loop(c);
     if (states)
         for (int pi = 0; pi < c->nprob; ++pi) to_time_state(c->hst[pi], &states[pi]);
@@ -2145,6 +2315,59 @@ int lulesh_hip_batch_run(lulesh_ctx *c, int32_t max_cycles, lulesh_time_state *s
 }
 
 double lulesh_hip_last_run_ms(const lulesh_ctx *c) { return c ? c->last_run_ms : 0.0; }
+
+int lulesh_hip_stream_bandwidth(int64_t bytes, int32_t reps, double *GBs) {
+    if (bytes < (int64_t(1) << 20) || reps < 1 || !GBs) return LULESH_INVALID_ARGUMENT;
+    *GBs = 0;
+    const int64_t n = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    auto fail = [&](hipError_t e) {
+        if (rc == 0 && e != hipSuccess) rc = LULESH_HIP_ERROR - int(e);
+        return rc != 0;
+    };
+    int dev = 0, cus = 256;
+    if (!fail(hipGetDevice(&dev)) &&
+        !fail(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) &&
+        !fail(hipMalloc(&a, size_t(n) * 16)) && !fail(hipMalloc(&b, size_t(n) * 16)) &&
+        !fail(hipMemset(a, 1, size_t(n) * 16)) &&
+        !fail(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) &&
+        !fail(hipEventCreate(&e0)) && !fail(hipEventCreate(&e1))) {
+        const vec2d *src = static_cast<const vec2d *>(a);
+        vec2d *dst = static_cast<vec2d *>(b);
+        // launch shapes: one pass per lane (1, 2 or 4 vectors), and a grid of
+        // 8 or 16 workgroups per CU walking the buffer with 4 vectors
+        for (int shape = 0; shape < 5 && rc == 0; ++shape) {
+            auto go = [&] {
+                switch (shape) {
+                case 0: k_stream_copy<1, false><<<grid_for(n, 256), 256, 0, st>>>(src, dst, n); break;
+                case 1: k_stream_copy<2, false><<<grid_for(n, 512), 256, 0, st>>>(src, dst, n); break;
+                case 2: k_stream_copy<4, false><<<grid_for(n, 1024), 256, 0, st>>>(src, dst, n); break;
+                case 3: k_stream_copy<4, true><<<8 * cus, 256, 0, st>>>(src, dst, n); break;
+                default: k_stream_copy<4, true><<<16 * cus, 256, 0, st>>>(src, dst, n); break;
+                }
+            };
+            go(); // warm-up
+            if (fail(hipEventRecord(e0, st))) break;
+            for (int r = 0; r < reps; ++r) go();
+            if (fail(hipGetLastError()) || fail(hipEventRecord(e1, st)) ||
+                fail(hipEventSynchronize(e1)))
+                break;
+            float ms = 0;
+            if (fail(hipEventElapsedTime(&ms, e0, e1))) break;
+            if (ms > 0) *GBs = std::max(*GBs, 2.0 * double(n) * 16 * reps / (ms * 1e-3) / 1e9);
+        }
+    }
+    if (st) (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}
 
 int lulesh_hip_download(lulesh_ctx *c, int id, void *dst, size_t count) {
     return field_copy(c, 0, id, dst, count, true);
@@ -2167,7 +2390,7 @@ int lulesh_hip_download_view(lulesh_ctx *c, lulesh_domain_view *dst) {
     if (!c || !dst || c->nprob != 1) return LULESH_INVALID_ARGUMENT;
     for (int id = 0; id < LULESH_F_COUNT; ++id) {
         double *p = view_field(dst, id);
-        if (!p) continue;
+        if (!p || (is_force_field(id) && !c->store_forces)) continue;
         int rc = lulesh_hip_download(c, id, p, is_node_field(id) ? size_t(c->totN)
                                                                  : size_t(c->totE));
         if (rc) return rc;
@@ -2287,7 +2510,7 @@ int lulesh_hip_autotune(lulesh_ctx *c, int32_t cycles, lulesh_tuning *best,
         for (size_t l = 0; l < L && r == 0; ++l) {
             cs[l]->tune = t;
             if (!(r = configure(cs[l])) && !(r = rearm(l)))
-                r = build_graph(cs[l]); // graph build outside the timing
+                r = build_graphs(cs[l]); // graph build outside the timing
         }
         // a first run warms the new graphs and launch tables up; the second,
         // from the same snapshot, is the one timed (device time of the
@@ -2420,7 +2643,7 @@ void lulesh_hip_destroy(lulesh_ctx *c) {
     if (!c) return;
     for (lulesh_ctx *l : c->lanes) lulesh_hip_destroy(l);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    destroy_graphs(c);
     for (void *p : c->allocs) (void)hipFree(p);
     for (void *p : {(void *)c->d_pd, (void *)c->d_segs, (void *)c->d_zBlockProb,
                     (void *)c->d_eBlockProb, (void *)c->d_mBlockSeg, (void *)c->d_oBlockProb})

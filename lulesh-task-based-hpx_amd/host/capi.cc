@@ -148,6 +148,9 @@ int32_t lulesh_hip_is_structured(const lulesh_ctx *c) {
 double lulesh_hip_last_run_ms(const lulesh_ctx *c) {
     return LULESH_FWD0(c, lulesh_hip_last_run_ms);
 }
+int lulesh_hip_stream_bandwidth(int64_t bytes, int32_t reps, double *GBs) {
+    return ex::lulesh_hip_stream_bandwidth(bytes, reps, GBs);
+}
 int lulesh_hip_set_tuning(lulesh_ctx *c, const lulesh_tuning *tuning) {
     return LULESH_FWD(c, lulesh_hip_set_tuning, tuning);
 }

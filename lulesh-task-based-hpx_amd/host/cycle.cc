@@ -158,6 +158,11 @@ int lulesh_domain_scale_energy(lulesh_domain *d, double factor) {
     return 0;
 }
 
+int lulesh_domain_write_viz(lulesh_domain *d, const char *dir, int32_t pieces) {
+    if (!d || pieces < 1) return LULESH_INVALID_ARGUMENT;
+    return lulesh::WriteViz(lulesh::VizOf(d->dom), dir, pieces);
+}
+
 int lulesh_batch_params(uint64_t seed, int32_t nprob, int32_t smin, int32_t smax,
                         int32_t *sizes, int32_t *regions, double *factors) {
     if (nprob < 0 || smin < 1 || smax < smin || !sizes || !regions || !factors)

@@ -219,4 +219,24 @@ void VerifyNumbers(Domain &domain, Real_t out[4]);
 
 Int_t RegionRep(Int_t reg, Int_t numReg, Int_t cost);
 
+// The arrays of the visualization dump (lulesh-viz.cc:56-197) of one
+// structured Sedov cube of nx elements per edge.  nodelist may be NULL (the
+// structured connectivity of lulesh-init.cc:236-244), regNumList too (every
+// element in region 1).
+struct VizFields {
+    Index_t nx = 0;
+    Int_t cycle = 0;
+    Real_t time = 0;
+    const Real_t *x = nullptr, *y = nullptr, *z = nullptr, *xd = nullptr, *yd = nullptr,
+                 *zd = nullptr;
+    const Real_t *e = nullptr, *p = nullptr, *v = nullptr, *q = nullptr;
+    const Index_t *regNumList = nullptr, *nodelist = nullptr;
+};
+// Write lulesh_plot_c<cycle>.vtk into directory `dir` (NULL or "" = the
+// working directory), or `pieces` slabs and a .visit list.  0 on success,
+// -3 bad arguments, -4 a file could not be written.
+int WriteViz(const VizFields &f, const char *dir, Int_t pieces);
+// The dump arrays of a host Domain (its current fields and time state).
+VizFields VizOf(Domain &d);
+
 } // namespace lulesh

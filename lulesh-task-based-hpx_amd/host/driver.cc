@@ -19,8 +19,12 @@
 //   --device-setup  build the problem in HBM (lulesh_hip_create_sedov)
 //                 instead of on the host; the report comes from the device
 //                 fields (not with --host-loop)
-// Single-dash spellings (-s 30) are accepted too.  --f and --v (VisIt
-// output) are not supported.
+//   --v       write the final state for VisIt (lulesh.cc:2397-2400): VTK
+//             legacy files lulesh_plot_c<cycle>.vtk in the working directory
+//             (the reference's Silo dump, lulesh-viz.cc, has the same objects)
+//   --f <n>   split that dump into n slabs of element planes, listed in
+//             lulesh_plot_c<cycle>.visit (default 1)
+// Single-dash spellings (-s 30) are accepted too.
 #include <sys/time.h>
 
 #include <cstdio>
@@ -29,6 +33,7 @@
 #include <iomanip>
 #include <iostream>
 #include <string>
+#include <vector>
 
 #include "domain.hpp"
 
@@ -38,6 +43,7 @@ namespace {
 
 struct Opts {
     Int_t its = 9999999, nx = 30, numReg = 11, showProg = 0, quiet = 0, balance = 1, cost = 1;
+    Int_t numFiles = 1, viz = 0;
     int precision = LULESH_PRECISION_FP64;
     bool hostLoop = false, deviceSetup = false, fma = false;
 };
@@ -45,7 +51,8 @@ struct Opts {
 [[noreturn]] void usage(const char *msg) {
     std::fprintf(stderr, "%s\n", msg);
     std::fprintf(stderr,
-                 "usage: lulesh-amd [--s n] [--i n] [--r n] [--b n] [--c n] [--q] [--p]\n"
+                 "usage: lulesh-amd [--s n] [--i n] [--r n] [--b n] [--c n] [--q] [--p] [--v]\n"
+                 "                  [--f n]\n"
                  "                  [--precision fp64|mixed] [--fma] [--host-loop] [--device-setup]\n");
     std::exit(-1);
 }
@@ -71,10 +78,10 @@ Opts parse(int argc, char **argv) {
         else if (k == "r") o.numReg = num();
         else if (k == "b") o.balance = num();
         else if (k == "c") o.cost = num();
-        else if (k == "f") (void)num(); // number of VisIt files: accepted, unused
+        else if (k == "f") o.numFiles = num();
         else if (k == "q") o.quiet = 1;
         else if (k == "p") o.showProg = 1;
-        else if (k == "v") usage("Use of -v requires compiling with -DVIZ_MESH");
+        else if (k == "v") o.viz = 1;
         else if (k == "host-loop") o.hostLoop = true;
         else if (k == "device-setup") o.deviceSetup = true;
         else if (k == "fma") o.fma = true;
@@ -87,6 +94,7 @@ Opts parse(int argc, char **argv) {
         else usage(("unknown option " + std::string(argv[a])).c_str());
     }
     if (o.nx < 1 || o.numReg < 1) usage("--s and --r must be >= 1");
+    if (o.numFiles < 1) usage("--f must be >= 1");
     if (o.hostLoop && o.deviceSetup) usage("--host-loop needs the host set-up");
     return o;
 }
@@ -102,6 +110,46 @@ void check(int rc, lulesh_ctx *ctx) {
     if (rc == VolumeError || rc == QStopError) std::exit(rc); // lulesh.cc:821, 908, 1262
     std::fprintf(stderr, "lulesh-amd: device error %d: %s\n", rc, lulesh_hip_last_error(ctx));
     std::exit(-1);
+}
+
+// The viz dump of a problem built in HBM: its fields come back from the
+// device, the region numbers from the region runs of the set-up.
+void write_viz_from_device(const Opts &o, lulesh_ctx *ctx, const lulesh_time_state &ts) {
+    const int64_t E = int64_t(o.nx) * o.nx * o.nx, N = int64_t(o.nx + 1) * (o.nx + 1) * (o.nx + 1);
+    std::vector<Real_t> nf[6], ef[4];
+    const int nid[6] = {LULESH_F_X, LULESH_F_Y, LULESH_F_Z, LULESH_F_XD, LULESH_F_YD, LULESH_F_ZD};
+    const int eid[4] = {LULESH_F_E, LULESH_F_P, LULESH_F_V, LULESH_F_Q};
+    for (int k = 0; k < 6; ++k) {
+        nf[k].resize(size_t(N));
+        check(lulesh_hip_download(ctx, nid[k], nf[k].data(), size_t(N)), ctx);
+    }
+    for (int k = 0; k < 4; ++k) {
+        ef[k].resize(size_t(E));
+        check(lulesh_hip_download(ctx, eid[k], ef[k].data(), size_t(E)), ctx);
+    }
+    std::vector<RegionRun> runs;
+    SedovRegionRuns(Index_t(E), o.numReg, o.balance, runs);
+    std::vector<Index_t> reg(static_cast<size_t>(E));
+    for (size_t r = 0; r < runs.size(); ++r) {
+        const int64_t end = r + 1 < runs.size() ? runs[r + 1].start : E;
+        for (int64_t e = runs[r].start; e < end; ++e) reg[size_t(e)] = runs[r].region;
+    }
+    VizFields f;
+    f.nx = o.nx;
+    f.cycle = ts.cycle;
+    f.time = ts.time;
+    f.x = nf[0].data();
+    f.y = nf[1].data();
+    f.z = nf[2].data();
+    f.xd = nf[3].data();
+    f.yd = nf[4].data();
+    f.zd = nf[5].data();
+    f.e = ef[0].data();
+    f.p = ef[1].data();
+    f.v = ef[2].data();
+    f.q = ef[3].data();
+    f.regNumList = reg.data();
+    if (WriteViz(f, nullptr, o.numFiles) != 0) std::printf("Error writing out viz file - rank 0\n");
 }
 
 // VerifyAndWriteFinalOutput, lulesh-util.cc:60-115; v = e(0), MaxAbsDiff,
@@ -164,6 +212,7 @@ int main(int argc, char **argv) {
         const double elapsed = now_s() - t0;
         Real_t v[4];
         check(lulesh_hip_report(ctx, v), ctx);
+        if (o.viz) write_viz_from_device(o, ctx, ts);
         lulesh_hip_destroy(ctx);
         if (!o.quiet)
             report(v, ts.cycle, elapsed, o.nx);
@@ -176,7 +225,11 @@ int main(int argc, char **argv) {
     Domain dom(o.nx, o.numReg, o.balance, o.cost);
     lulesh_domain_view view = dom.view();
     lulesh_ctx *ctx = nullptr;
-    check(lulesh_hip_create(&view, o.precision | (o.fma ? LULESH_OPT_FMA : 0), &ctx), ctx);
+    // the report needs e only: the dead node forces and accelerations are not stored
+    check(lulesh_hip_create(&view,
+                            o.precision | (o.fma ? LULESH_OPT_FMA : 0) | LULESH_OPT_NO_NODE_FORCES,
+                            &ctx),
+          ctx);
 
     // Timed window as in the reference (lulesh.cc:2368-2395): the cycle loop
     // only, with the fields already resident on the device.
@@ -218,6 +271,10 @@ int main(int argc, char **argv) {
     dom.time() = keepTime;
     dom.cycle() = keepCycle;
     lulesh_hip_destroy(ctx);
+
+    // Write out the final viz file (lulesh.cc:2397-2400)
+    if (o.viz && WriteViz(VizOf(dom), nullptr, o.numFiles) != 0)
+        std::printf("Error writing out viz file - rank 0\n");
 
     if (!o.quiet) {
         Real_t v[4];

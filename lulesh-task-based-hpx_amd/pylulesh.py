@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "liblulesh_amd.so")
 
 OK, VOLUME_ERROR, QSTOP_ERROR, INVALID_ARGUMENT = 0, -1, -2, -3
 FP64, MIXED = 0, 1
-OPT_NODE_FORCES, OPT_GENERIC, OPT_FMA = 0x100, 0x200, 0x400
+OPT_NODE_FORCES, OPT_GENERIC, OPT_FMA, OPT_NO_NODE_FORCES = 0x100, 0x200, 0x400, 0x800
 
 
 def OPT_STREAMS(k):
@@ -129,6 +129,8 @@ def lib():
         "lulesh_hip_batch_create_sedov": (ctypes.c_int, [_I, _I, _D, i32, i32, i32, ctypes.c_int,
                                                          ctypes.POINTER(vp)]),
         "lulesh_hip_report": (ctypes.c_int, [vp, _D]),
+        "lulesh_hip_stream_bandwidth": (ctypes.c_int, [ctypes.c_int64, i32, _D]),
+        "lulesh_domain_write_viz": (ctypes.c_int, [vp, ctypes.c_char_p, i32]),
     }
     for name, (res, args) in more.items():
         fn = getattr(L, name)
@@ -208,6 +210,12 @@ class Domain:
 
     def refresh_view(self):
         self.L.lulesh_domain_get_view(self.h, ctypes.byref(self.view))
+
+    def write_viz(self, directory, pieces=1):
+        """The -v dump of the current host fields (lulesh_domain_write_viz)."""
+        rc = self.L.lulesh_domain_write_viz(self.h, os.fsencode(directory), pieces)
+        if rc != 0:
+            raise LuleshError(rc, "lulesh_domain_write_viz")
 
 
 class Device:

@@ -217,3 +217,41 @@ def test_stream_group_host_domains_and_tuning(pyl):
         for name in FIELDS:
             np.testing.assert_array_equal(grp.download(p, name), dev.download(name),
                                           err_msg=f"problem {p} {name}")
+
+
+def test_benchmark_batch_of_64(pyl):
+    """The benchmark ensemble itself: 64 problems drawn with seed 0 (sizes
+    in [20, 80], regions in {11, 16, 21}, deposit x [0.5, 2)), 1000 cycles
+    or each problem's stop time, in the benchmark's form (multiply-adds
+    fused, two streams).  Every problem ends bit for bit where the same
+    problem run alone ends; three of them (the smallest mesh, which has 21
+    regions; the largest; a 21-region s=52 one) match the oracle's
+    full-length runs (tests/golden/oracle_batch64_p*.json) at 1e-9."""
+    from test_gpu_long import compare_with_fixture, load_fixture
+
+    banner("benchmark batch: 64 problems, 1000 cycles")
+    sizes, regions, factors = pyl.batch_params(0, 64)
+    its = 1000
+    b = pyl.Batch.sedov(sizes, regions, factors, opts=pyl.OPT_FMA | pyl.OPT_STREAMS(2))
+    rc, states = b.run(its)
+    assert rc == 0
+    cyc = [st.cycle for st in states]
+    print(f"[batch64] cycles {min(cyc)}..{max(cyc)}", flush=True)
+    assert max(cyc) == its and min(cyc) < its  # the small problems stop early
+    for k in range(64):
+        one = pyl.Batch.sedov([sizes[k]], [regions[k]], [factors[k]], opts=pyl.OPT_FMA)
+        rc1, st1 = one.run(its)
+        assert rc1 == 0
+        assert (st1[0].cycle, st1[0].time, st1[0].deltatime) == \
+            (states[k].cycle, states[k].time, states[k].deltatime), k
+        for name in FIELDS:
+            np.testing.assert_array_equal(b.download(k, name), one.download(0, name),
+                                          err_msg=f"problem {k} {name}")
+        one.close()
+    for k in (40, 17, 26):
+        meta, arrays = load_fixture(f"batch64_p{k}")
+        assert (meta["s"], meta["nr"]) == (int(sizes[k]), int(regions[k]))
+        assert meta["factor"] == float(factors[k])
+        print(f"[batch64] problem {k}: s={meta['s']} regions={meta['nr']}", flush=True)
+        compare_with_fixture(meta, arrays, lambda f: b.download(k, f), states[k], 1e-9)
+    b.close()

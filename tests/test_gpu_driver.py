@@ -59,3 +59,37 @@ def test_driver_fma_form():
     assert "fused multiply-add" in f
     fd = run("--s", "30", "--fma", "--device-setup")
     assert energy17(fd) == energy17(f)
+
+
+def test_driver_viz_dump(tmp_path):
+    """--v writes the final state for VisIt (lulesh.cc:2397-2400) in the
+    working directory: the host set-up and the set-up in HBM give the same
+    file (region numbers included), whole or split by --f; its energies are
+    the run's (the 17-digit origin energy of the report, as float)."""
+    from test_host_viz import read_vtk
+
+    def run_in(d, *args):
+        print(f"\n[driver] (in {d.name}) {' '.join(args)}", flush=True)
+        r = subprocess.run([EXE, *args], capture_output=True, text=True, timeout=240, cwd=d)
+        assert r.returncode == 0, r.stderr
+        return r.stdout
+
+    a, b, c = (tmp_path / n for n in ("host", "device", "split"))
+    for d in (a, b, c):
+        d.mkdir()
+    out = run_in(a, "--s", "6", "--i", "40", "--v")
+    run_in(b, "--s", "6", "--i", "40", "--v", "--device-setup")
+    run_in(c, "--s", "6", "--i", "40", "--v", "--f", "3")
+    assert os.listdir(a) == ["lulesh_plot_c40.vtk"]
+    va = (a / "lulesh_plot_c40.vtk").read_bytes()
+    assert (b / "lulesh_plot_c40.vtk").read_bytes() == va
+    whole = read_vtk(a / "lulesh_plot_c40.vtk")
+    assert whole["fields"]["CYCLE"][0] == 40 and whole["cells"].shape == (216, 9)
+    e0 = float(energy17(out))
+    assert whole["cell"]["e"][0] == pytest.approx(e0, rel=1e-7)
+    assert len(set(whole["cell"]["regions"].tolist())) > 1
+    assert (c / "lulesh_plot_c40.visit").read_text().startswith("!NBLOCKS 3\n")
+    parts = [read_vtk(c / f"lulesh_plot_c40.{k:03d}.vtk") for k in range(3)]
+    np_e = [p["cell"]["e"] for p in parts]
+    import numpy as np
+    np.testing.assert_array_equal(np.concatenate(np_e), whole["cell"]["e"])

@@ -1,0 +1,115 @@
+"""Full-length runs of the benchmark configurations against the oracle.
+
+The oracle fixtures (tests/golden/oracle_*.json + .npz, written by
+tests/golden/make_oracle_fixtures.py from the CPU oracle) hold the final
+state of
+
+  * s=150, 11 regions, -b 1 -c 1, 2000 cycles (the headline problem), and
+  * s=250, 21 regions, -b 1 -c 1, 500 cycles (region imbalance at scale):
+
+the time state, the report numbers with 17 digits, per state field its
+max |f|, sum f, sum |f| and sum f^2, and the field values at fixed indices
+(the cube of elements / nodes next to the origin, where the blast is, and a
+seeded random sample of the rest).
+
+Each FP64 arithmetic form (every operation rounded on its own, and
+multiply-adds fused) must reach the same cycle and time, the origin energy
+within 1e-9 relative, every sampled value within 1e-9 of the field's scale
+(max |f| of the oracle), every field sum within 1e-9 of its sum |f|, and
+plane-0 symmetry diffs no worse than the oracle's beyond rounding noise.
+The FP32-mixed form: the same cycle, origin energy within 1e-6.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+NODE = ["x", "y", "z", "xd", "yd", "zd"]
+ELEM = ["e", "p", "q", "ql", "qq", "v", "ss", "delv", "vdov", "arealg", "vnew"]
+
+
+def load_fixture(name):
+    with open(os.path.join(GOLDEN, f"oracle_{name}.json")) as fh:
+        meta = json.load(fh)
+    with np.load(os.path.join(GOLDEN, f"oracle_{name}.npz")) as z:
+        arrays = {k: z[k] for k in z.files}
+    return meta, arrays
+
+
+def banner(msg):
+    print(f"\n[long] {msg}", flush=True)
+
+
+def compare_with_fixture(meta, arrays, get_field, ts, tol):
+    """Worst relative differences of a final state against an oracle
+    fixture: the cycle and time, the origin energy, and every state field
+    (sampled values, sums, max |f|) relative to the field's scale; asserts
+    them at `tol`."""
+    assert ts.cycle == meta["cycle"], (ts.cycle, meta["cycle"])
+    assert abs(ts.time - meta["time"]) <= 1e-12 * meta["time"]
+    e0 = meta["report"]["e0"]
+    e0_gpu = float(get_field("e")[0])
+    e0_rel = abs(e0_gpu - e0) / abs(e0)
+    worst = {}
+    for f in NODE + ELEM:
+        a = get_field(f)
+        st = meta["stats"][f]
+        scale = max(st["max_abs"], 1e-300)
+        idx = arrays["node_index" if f in NODE else "elem_index"]
+        d_sample = float(np.max(np.abs(a[idx] - arrays[f]))) / scale
+        d_sum = abs(float(np.sum(a)) - st["sum"]) / max(st["sum_abs"], 1e-300)
+        d_max = abs(float(np.max(np.abs(a))) - st["max_abs"]) / scale
+        worst[f] = max(d_sample, d_sum, d_max)
+    print(f"[long] e0 {e0_gpu!r} oracle {e0!r} rel {e0_rel:.2e}; worst field "
+          f"{max(worst, key=worst.get)} {max(worst.values()):.2e}", flush=True)
+    assert e0_rel <= tol
+    for f, w in worst.items():
+        assert w <= tol, (f, w)
+    return e0_rel
+
+
+FORMS = {"exact": (0, 0), "fused": (0, 0x400), "mixed": (1, 0x400)}
+
+
+@pytest.mark.parametrize("form", ["exact", "fused", "mixed"])
+@pytest.mark.parametrize("name", ["s150_r11_i2000", "s250_r21_i500"])
+def test_full_length_run_vs_oracle(pyl, name, form):
+    meta, arrays = load_fixture(name)
+    prec, opts = FORMS[form]
+    banner(f"{name} {form}: {meta['its']} cycles vs the oracle fixture")
+    dev = pyl.Device.sedov(meta["s"], meta["nr"], meta["balance"], meta["cost"],
+                           precision=prec, opts=opts)
+    ts = dev.time_state()
+    assert dev.run(meta["its"], ts) == 0
+    rep = dev.report()
+    fields = {}
+
+    def get(f):
+        if f not in fields:
+            fields[f] = dev.download(f)
+        return fields[f]
+
+    if form == "mixed":
+        # FP32 hourglass and Q gradients: the north-star bound on the origin
+        # energy, and the symmetry of plane 0 at FP32 level
+        assert ts.cycle == meta["cycle"]
+        e0 = meta["report"]["e0"]
+        rel = abs(rep[0] - e0) / abs(e0)
+        print(f"[long] mixed e0 rel err vs oracle {rel:.2e}, MaxRelDiff {rep[3]:.2e}",
+              flush=True)
+        assert rel <= 1e-6
+        assert rep[3] <= 1e-3
+    else:
+        compare_with_fixture(meta, arrays, get, ts, 1e-9)
+        orep = meta["report"]
+        print(f"[long] report {list(map(float, rep))} oracle {orep}", flush=True)
+        assert rep[0] == get("e")[0]
+        # symmetry no worse than the oracle's: both are rounding noise of the
+        # same size, so a factor 4 of headroom
+        assert rep[1] <= max(4 * orep["MaxAbsDiff"], 1e-13 * abs(orep["e0"]))
+        assert rep[3] <= max(4 * orep["MaxRelDiff"], 1e-13)
+    dev.close()

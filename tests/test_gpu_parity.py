@@ -333,3 +333,75 @@ def test_mixed_precision_error(pyl, s, its):
     print("mixed rel err", abs(e0 - o0) / abs(o0))
     assert abs(e0 - o0) <= 1e-6 * abs(o0)
     assert ts.cycle == ora.cycle
+
+
+@pytest.mark.parametrize("path", ["structured", "generic"])
+def test_elements_phase_reads_the_newest_node_state(pyl, path):
+    """The elements phase called without a nodal phase before it reads the
+    node state the context holds: (1) straight after an upload of the
+    oracle's state into a fresh context, (2) after a device run.  Each is
+    compared bit for bit with the oracle's elements phase on the same
+    state."""
+    s = 8
+    banner(f"elements phase on its own, {path}")
+    ora = OracleDomain(s)
+    assert ora.run(25) == 0
+    dom = pyl.Domain(s)
+    dev = pyl.Device(dom, opts=PATHS[path])
+    copy_state(pyl, ora, dev)
+    ora.time_increment()
+    dt = ora.scalar("deltatime")
+    assert dev.phase(pyl.PHASE_ELEMENTS, dt) == 0
+    assert ora.elements() == 0
+    out = ["vnew", "delv", "arealg", "vdov", "v", "delv_xi", "delv_eta", "delv_zeta",
+           "delx_xi", "delx_eta", "delx_zeta"]
+    for f in out:
+        np.testing.assert_array_equal(dev.download(f), ora.f64(f), err_msg=f"upload: {f}")
+    dev.close()
+    # (2) a device run, then the phase: the oracle gets the device's state
+    dom2 = pyl.Domain(s)
+    dev2 = pyl.Device(dom2, opts=PATHS[path])
+    ts = dom2.time_state()
+    assert dev2.run(20, ts) == 0
+    ora2 = OracleDomain(s)
+    for f in pyl.FIELDS:
+        if f not in ("fx", "fy", "fz", "xdd", "ydd", "zdd"):
+            ora2.f64(f)[:] = dev2.download(f)
+    dt2 = 0.9 * ts.deltatime
+    assert dev2.phase(pyl.PHASE_ELEMENTS, dt2) == 0
+    ora2.set_scalar("deltatime", dt2)
+    assert ora2.elements() == 0
+    for f in out:
+        np.testing.assert_array_equal(dev2.download(f), ora2.f64(f), err_msg=f"after run: {f}")
+    # and the nodal phase after it starts from the same node state
+    assert dev2.phase(pyl.PHASE_NODAL, dt2) == 0
+    assert ora2.nodal() == 0
+    for f in ("x", "xd", "fx", "xdd"):
+        assert max_rel(dev2.download(f), ora2.f64(f)) < 1e-13, f
+    dev2.close()
+
+
+def test_node_forces_kept_by_default(pyl):
+    """A context made from a host Domain keeps fx.. and xdd.. current as the
+    reference's Domain does; with LULESH_OPT_NO_NODE_FORCES their downloads
+    are refused and download_view leaves the host arrays alone."""
+    s = 6
+    banner("node forces: default and opt-out")
+    ora = OracleDomain(s)
+    assert ora.run(12) == 0
+    dom, dev, ts = pyl.run_device(s, 12)
+    for f in ("fx", "fy", "fz", "xdd", "ydd", "zdd"):
+        assert max_rel(dom.field(f), ora.f64(f)) < 1e-12, f
+        assert np.any(dom.field(f) != 0), f
+    dev.close()
+    dom2 = pyl.Domain(s)
+    dev2 = pyl.Device(dom2, opts=pyl.OPT_NO_NODE_FORCES)
+    ts2 = dom2.time_state()
+    assert dev2.run(12, ts2) == 0
+    with pytest.raises(pyl.LuleshError):
+        dev2.download("fx")
+    dom2.field("fx")[:] = 7.0
+    dev2.download_view()
+    assert np.all(dom2.field("fx") == 7.0)
+    np.testing.assert_array_equal(dom2.field("e"), dom.field("e"))
+    dev2.close()

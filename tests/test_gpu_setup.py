@@ -30,7 +30,8 @@ def test_device_setup_equals_host_setup(pyl, s, nr, b, c, precision):
     prec = pyl.FP64 if precision == "fp64" else pyl.MIXED
     dom = pyl.Domain(s, nr, b, c)
     host = pyl.Device(dom, prec)
-    dev = pyl.Device.sedov(s, nr, b, c, prec)
+    # (the host Domain's context keeps fx.. / xdd.. by default; this one asks)
+    dev = pyl.Device.sedov(s, nr, b, c, prec, opts=pyl.OPT_NODE_FORCES)
     assert dev.structured()
     for f in FIELDS_ALL:
         np.testing.assert_array_equal(dev.download(f), host.download(f), err_msg=f)
@@ -76,8 +77,8 @@ def test_batch_setup_in_hbm_equals_host_batch(pyl):
     banner("batch of 12")
     sizes, regions, factors = pyl.batch_params(3, 12, 5, 30)
     doms = [pyl.make_domain(int(s), int(r), float(f)) for s, r, f in zip(sizes, regions, factors)]
-    host = pyl.Batch(doms)
-    dev = pyl.Batch.sedov(sizes, regions, factors)
+    host = pyl.Batch(doms, opts=pyl.OPT_NODE_FORCES)
+    dev = pyl.Batch.sedov(sizes, regions, factors, opts=pyl.OPT_NODE_FORCES)
     for k, s in enumerate(sizes):
         for f in FIELDS_ALL:
             np.testing.assert_array_equal(dev.download(k, f), host.download(k, f),

@@ -129,6 +129,41 @@ def test_create_rejects_bad_views_without_device(pyl):
     assert L.lulesh_hip_report(None, None) == pyl.INVALID_ARGUMENT
 
 
+def test_create_rejects_region_lists_that_are_not_a_partition(pyl):
+    """Region lists with a duplicated, missing or out-of-range element are
+    refused before any device call (the material kernels need every element
+    in exactly one region, as lulesh-init.cc:341-445 builds them)."""
+    L = pyl.lib()
+    h = ctypes.c_void_p()
+    dom = pyl.Domain(4, 11)
+    sizes = [int(dom.view.regElemSize[r]) for r in range(11)]
+    r = max(range(11), key=lambda k: sizes[k])  # a region with at least two elements
+    lst = dom.view.regElemlist[r]
+    keep0, keep1 = lst[0], lst[1]
+    try:
+        lst[1] = keep0                       # duplicate: element keep1 has no region
+        assert L.lulesh_hip_create(ctypes.byref(dom.view), 0, ctypes.byref(h)) == \
+            pyl.INVALID_ARGUMENT
+        lst[1] = 64                          # out of range (E = 64)
+        assert L.lulesh_hip_create(ctypes.byref(dom.view), 0, ctypes.byref(h)) == \
+            pyl.INVALID_ARGUMENT
+        lst[1] = keep1
+        dom.view.regElemSize[r] = sizes[r] - 1  # element keep-last missing
+        assert L.lulesh_hip_create(ctypes.byref(dom.view), 0, ctypes.byref(h)) == \
+            pyl.INVALID_ARGUMENT
+    finally:
+        lst[1] = keep1
+        dom.view.regElemSize[r] = sizes[r]
+    # keeping and not keeping the node forces at once
+    both = pyl.OPT_NODE_FORCES | pyl.OPT_NO_NODE_FORCES
+    assert L.lulesh_hip_create(ctypes.byref(dom.view), both, ctypes.byref(h)) == \
+        pyl.INVALID_ARGUMENT
+    # not keeping them is an option of lulesh_hip_create only (off already elsewhere)
+    arr = (ctypes.POINTER(pyl.DomainView) * 1)(ctypes.pointer(dom.view))
+    assert L.lulesh_hip_batch_create(arr, 1, pyl.OPT_NO_NODE_FORCES, ctypes.byref(h)) == \
+        pyl.INVALID_ARGUMENT
+
+
 def test_driver_flag_errors():
     exe = os.path.join(ROOT, "lulesh-task-based-hpx_amd", "lulesh-amd")
     if not os.path.exists(exe):
