@@ -100,6 +100,8 @@ __device__ __forceinline__ float sqrt_area(float x) { return sqrt_(x); }
 // as equal to the previous one: the value is declared modified by an empty
 // asm statement (no instruction is emitted).
 __device__ __forceinline__ void opaque(double &v) { asm volatile("" : "+v"(v)); }
+template <typename T>
+__device__ __forceinline__ void opaque_t(T &v) { asm volatile("" : "+v"(v)); }
 
 // Store of a per-element result.  NT: with the non-temporal hint (`nt` on
 // the store), so the written lines are not kept in the L2 for reuse.
@@ -279,6 +281,77 @@ __device__ __forceinline__ constexpr double gamma_hg(int mode, int node) {
                                      {1, -1, 1, -1, 1, -1, 1, -1},
                                      {-1, 1, -1, 1, 1, -1, 1, -1}};
     return double(g[mode][node]);
+}
+
+// The hourglass mode projections of the node coordinates, hourmodx/y/z of
+// lulesh.cc:962-978 (products with +-1 are exact, so the running sum is the
+// reference's left-to-right sum).
+template <typename T>
+__device__ __forceinline__ void hourglass_modes(const T x[8], const T y[8], const T z[8],
+                                                T hmx[4], T hmy[4], T hmz[4]) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        T hx = x[0] * T(gamma_hg(m, 0)), hy = y[0] * T(gamma_hg(m, 0)),
+          hz = z[0] * T(gamma_hg(m, 0));
+#pragma unroll
+        for (int j = 1; j < 8; ++j) {
+            hx = hx + x[j] * T(gamma_hg(m, j));
+            hy = hy + y[j] * T(gamma_hg(m, j));
+            hz = hz + z[j] * T(gamma_hg(m, j));
+        }
+        hmx[m] = hx;
+        hmy[m] = hy;
+        hmz[m] = hz;
+    }
+}
+
+// Row i of hourgam (lulesh.cc:980-1022):
+// hourgam[i][m] = gamma[m][i] - volinv (dvdx_i hmx_m + dvdy_i hmy_m + dvdz_i hmz_m).
+template <typename T>
+__device__ __forceinline__ void hourgam_row(int i, T dx, T dy, T dz, const T hmx[4],
+                                            const T hmy[4], const T hmz[4], T volinv, T g[4]) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+        g[m] = T(gamma_hg(m, i)) - volinv * (dx * hmx[m] + dy * hmy[m] + dz * hmz[m]);
+}
+
+// CalcElemFBHourglassForce (lulesh.cc:527-562) in two passes over the
+// nodes: the first forms hxx (per velocity component, the reference's sum
+// over nodes 0..7 in order), the second each node's force (the reference's
+// four-term sum, times the coefficient), handed to sink(c, i, force).  The
+// hourgam rows are recomputed in the second pass (kept from being merged with
+// the first by an opaque copy of volinv), so besides the volume derivatives
+// only one row and the twelve hxx sums are live; the values are those of the
+// one-pass form, bit for bit.  vel(c, i) returns velocity component c of
+// node i.
+template <typename T, typename Vel, typename Sink>
+__device__ __forceinline__ void fb_hourglass_two_pass(const T dvdx[8], const T dvdy[8],
+                                                      const T dvdz[8], const T hmx[4],
+                                                      const T hmy[4], const T hmz[4], T volinv,
+                                                      T coefficient, Vel vel, Sink sink) {
+    T hxx[3][4];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        T g[4];
+        hourgam_row<T>(i, dvdx[i], dvdy[i], dvdz[i], hmx, hmy, hmz, volinv, g);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const T v = vel(c, i);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) hxx[c][m] = (i == 0) ? g[m] * v : hxx[c][m] + g[m] * v;
+        }
+    }
+    T vi2 = volinv;
+    opaque_t(vi2);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        T g[4];
+        hourgam_row<T>(i, dvdx[i], dvdy[i], dvdz[i], hmx, hmy, hmz, vi2, g);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            sink(c, i, coefficient * (g[0] * hxx[c][0] + g[1] * hxx[c][1] + g[2] * hxx[c][2] +
+                                      g[3] * hxx[c][3]));
+    }
 }
 
 // Flanagan-Belytschko hourglass force of one element: the hourglass modes

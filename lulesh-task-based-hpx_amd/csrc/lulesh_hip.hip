@@ -55,6 +55,10 @@
 #ifndef LULESH_IMPL_NS
 #define LULESH_IMPL_NS lulesh_impl_exact
 #endif
+// 1 in the build whose multiply-adds are fused (LULESH_OPT_FMA contexts)
+#ifndef LULESH_FUSED_FORM
+#define LULESH_FUSED_FORM 0
+#endif
 
 namespace LULESH_IMPL_NS {
 
@@ -66,6 +70,7 @@ using namespace lulesh_dev;
 namespace {
 
 constexpr int kBlock = 256; // default and maximum block of the element kernels
+constexpr bool kFusedForm = LULESH_FUSED_FORM != 0;
 
 __device__ __forceinline__ unsigned long long dbits(double v) {
     return (unsigned long long)__double_as_longlong(v);
@@ -949,6 +954,24 @@ double *view_field(const lulesh_domain_view *v, int id) {
 
 int grad_slot(int id) { return id - LULESH_F_DELV_XI; }
 
+// The node state x, y, z, xd, yd, zd: one allocation per half of the double
+// buffer (the structured path has two halves, the indexed path one), the six
+// fields totN doubles apart.  f[X..ZD] are the first half.
+int alloc_node_state(lulesh_ctx *c, bool two_halves) {
+    const size_t N = size_t(c->totN);
+    for (int h = 0; h < 2; ++h) {
+        if (h == 1 && !two_halves) {
+            for (int k = 0; k < 6; ++k) c->nb.b[1][k] = c->nb.b[0][k];
+            break;
+        }
+        double *block = nullptr;
+        if (int rc = dalloc(c, &block, 6 * N)) return rc;
+        for (int k = 0; k < 6; ++k) c->nb.b[h][k] = block + size_t(k) * N;
+    }
+    for (int k = 0; k < 6; ++k) c->f[LULESH_F_X + k] = c->nb.b[0][k];
+    return 0;
+}
+
 template <typename F>
 int launch(lulesh_ctx *c, int timer_id, F &&fn) {
     if (c->timing) {
@@ -1128,11 +1151,12 @@ int enqueue_nodal(lulesh_ctx *c) {
         const NodeForceOut fo{c->f[LULESH_F_FX], c->f[LULESH_F_FY], c->f[LULESH_F_FZ],
                               c->f[LULESH_F_XDD], c->f[LULESH_F_YDD], c->f[LULESH_F_ZDD]};
         return launch(c, T_NODAL, [&] {
+            constexpr int M = sizeof(R) == 4 ? MODE_MIXED : kFusedForm ? MODE_FUSED : MODE_EXACT;
             auto kern = c->zSquare16
-                            ? (c->store_forces ? k_nodal_zmarch<R, true, true>
-                                               : k_nodal_zmarch<R, false, true>)
-                            : (c->store_forces ? k_nodal_zmarch<R, true, false>
-                                               : k_nodal_zmarch<R, false, false>);
+                            ? (c->store_forces ? k_nodal_zmarch<M, true, true>
+                                               : k_nodal_zmarch<M, false, true>)
+                            : (c->store_forces ? k_nodal_zmarch<M, true, false>
+                                               : k_nodal_zmarch<M, false, false>);
             kern<<<c->zBlocks, kZmarchThreads, 0, c->stream>>>(
                 c->d_pd, c->d_zBlockProb, c->nb, fo, c->f[LULESH_F_P], c->f[LULESH_F_Q],
                 c->f[LULESH_F_VOLO], c->f[LULESH_F_V], c->f[LULESH_F_SS], c->f[LULESH_F_ELEMMASS],
@@ -1597,6 +1621,7 @@ int create_impl(const lulesh_domain_view *const *views, int nprob, int precision
         }
     }
     const size_t E = size_t(totE), N = size_t(totN);
+    if ((rc = alloc_node_state(c, structured))) return fail(rc);
     // Fields, concatenated over problems.
     for (int id = 0; id < LULESH_F_COUNT; ++id) {
         const size_t n = is_node_field(id) ? N : E;
@@ -1604,7 +1629,7 @@ int create_impl(const lulesh_domain_view *const *views, int nprob, int precision
             if ((rc = dalloc(c, &c->g32[grad_slot(id)], n))) return fail(rc);
             continue;
         }
-        if ((rc = dalloc(c, &c->f[id], n))) return fail(rc);
+        if (!is_state_node_field(id) && (rc = dalloc(c, &c->f[id], n))) return fail(rc);
         for (int pi = 0; pi < nprob; ++pi) {
             const ProbHost &ph = c->probs[size_t(pi)];
             const size_t off = is_node_field(id) ? size_t(ph.nodeOff) : size_t(ph.elemOff);
@@ -1612,11 +1637,6 @@ int create_impl(const lulesh_domain_view *const *views, int nprob, int precision
             if ((rc = upload(c, c->f[id] + off, (const double *)view_field(views[pi], id), cnt)))
                 return fail(rc);
         }
-    }
-    for (int k = 0; k < 6; ++k) {
-        c->nb.b[0][k] = c->f[LULESH_F_X + k];
-        c->nb.b[1][k] = c->f[LULESH_F_X + k];
-        if (structured && (rc = dalloc(c, &c->nb.b[1][k], N))) return fail(rc);
     }
     // Face neighbours and BC flags, shifted to concatenated element indices.
     {
@@ -1975,17 +1995,14 @@ int create_sedov_impl(const int32_t *sizes, const int32_t *regions, const double
     const size_t E = size_t(totE), N = size_t(totN);
     int rc;
     // every field zero-filled (the constructor's allocations), then set
+    if ((rc = alloc_node_state(c, true))) return fail(rc);
     for (int id = 0; id < LULESH_F_COUNT; ++id) {
         const size_t n = is_node_field(id) ? N : E;
         if (precision == LULESH_PRECISION_MIXED && is_grad_field(id)) {
             if ((rc = dalloc(c, &c->g32[grad_slot(id)], n))) return fail(rc);
             continue;
         }
-        if ((rc = dalloc(c, &c->f[id], n))) return fail(rc);
-    }
-    for (int k = 0; k < 6; ++k) {
-        c->nb.b[0][k] = c->f[LULESH_F_X + k];
-        if ((rc = dalloc(c, &c->nb.b[1][k], N))) return fail(rc);
+        if (!is_state_node_field(id) && (rc = dalloc(c, &c->f[id], n))) return fail(rc);
     }
     if ((rc = dalloc(c, &c->elemBC, E)) || (rc = dalloc(c, &c->erep, E))) return fail(rc);
     {

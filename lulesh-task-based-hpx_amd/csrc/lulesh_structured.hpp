@@ -117,19 +117,51 @@ struct ProbDesc {
     double inv_s, inv_ss;
 };
 
-// Two waves per SIMD: the FP64 form needs about 260 VGPRs with the register
-// prefetch of the next node plane; holding it to 256 costs a handful of
-// spilled values and keeps two workgroups per CU.
+// Node-plane ring in LDS: two slots, each holding x, y, z, xd, yd, zd of the
+// (pw+1) x (ph+1) patch nodes of one node plane (field f of patch node t at
+// slot[f * kPatchNodes + t]).
+//
+// Forms of the kernel:
+//  MODE_EXACT (FP64, every multiply and add rounded on its own): each
+//    corner's stress and hourglass force are staged separately and the node
+//    adds them one after the other, corner by corner, in the reference's
+//    interleaved order (lulesh.cc:849-857): bitwise the reference
+//    arithmetic.  24 staged values per element (77 KB of LDS); about 260
+//    VGPRs, held to 256 (a handful of spilled values): two workgroups per CU.
+//  MODE_FUSED (FP64, multiply-adds fused): the element adds its stress and
+//    hourglass force per corner before staging (one more rounding per
+//    corner, no change in what is summed), 12 staged values per element
+//    (52 KB), and forms the hourglass force in two register-lean passes:
+//    216 VGPRs without spills, half the LDS traffic of the node phases.  Two
+//    workgroups per CU.  At s=150 the nodal kernel takes 0.185 ms against
+//    0.211 ms staged as MODE_EXACT.
+//  MODE_MIXED (FP32 hourglass): staged as MODE_FUSED, 148 VGPRs and 52 KB:
+//    three workgroups per CU (0.178 ms against 0.212 ms staged separately).
+// Tried and dropped: FP64 staged combined at three workgroups per CU (needs
+// more than the 168 VGPRs of three waves per SIMD and spills: 0.48-0.59
+// ms); the next plane copied into LDS by global_load_lds during the element
+// work (the copies' waits and spills cost more than the latency they hide;
+// fetching the next plane into registers before the element work instead of
+// after it is slower too).
 // SQUARE16: every problem of the launch has 16 x 16 patches (the shape is
 // then a compile-time constant; runtime shapes cost 2-6% at s=150).
-template <typename HG, bool STORE_F, bool SQUARE16>
-__global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
+enum ZmarchMode { MODE_EXACT = 0, MODE_FUSED = 1, MODE_MIXED = 2 };
+
+template <int MODE>
+constexpr int zmarch_waves_per_eu() {
+    return MODE == MODE_MIXED ? 3 : 2;
+}
+
+template <int MODE, bool STORE_F, bool SQUARE16>
+__global__ __launch_bounds__(kZmarchThreads)
+__attribute__((amdgpu_waves_per_eu(zmarch_waves_per_eu<MODE>()))) void k_nodal_zmarch(
     const ProbDesc *__restrict__ pds, const int32_t *__restrict__ blockProb, NodeBufs nb,
     NodeForceOut fo0, const double *__restrict__ p0,
     const double *__restrict__ q0, const double *__restrict__ volo0,
     const double *__restrict__ v0, const double *__restrict__ ss0,
     const double *__restrict__ elemMass0, const double *__restrict__ nodalMass0, double hgcoef,
     double u_cut, DevState *states) {
+    constexpr bool COMBINE = MODE != MODE_EXACT; // corner sums staged (12 values)
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int prob = blockProb[blk];
     DevState *st = states + prob;
@@ -153,11 +185,13 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
     const double *__restrict__ elemMass = elemMass0 + eo;
     const double *__restrict__ nodalMass = nodalMass0 + no;
     const int lb = blk - pd.zBlock0;
-    // LDS: 4 corners x (3 stress + 3 hourglass) values per element of the
-    // patch, and the (pw+1) x (ph+1) patch nodes of the two node planes an
-    // element plane touches (a ring of two planes: x, y, z, xd, yd, zd each).
-    __shared__ double lds[24][kZmarchThreads];
-    __shared__ double pn[2][6][kPatchNodes];
+    // LDS: the staged corner values of the element plane (4 corners x 3
+    // stress + 3 hourglass values, or 3 sums), and the node-plane ring.
+    constexpr int kStage = COMBINE ? 12 : 24; // staged values per element
+    __shared__ double lds[kStage][kZmarchThreads];
+    __shared__ double ring[2][6 * kPatchNodes];
+    auto slot_of = [](int K) { return K & 1; };
+    auto pnv = [&](int slot, int f, int t) -> double & { return ring[slot][f * kPatchNodes + t]; };
 
     const int n = s + 1;
     const int tid = threadIdx.x;
@@ -188,8 +222,19 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
     const int own = (jj + dy) * (pw + 1) + (ii + dx); // node (J, I) in the patch node array
     auto finish_node = [&](int K, double mass) {
         const int g = (K * n + J) * n + I;
-        const int slot = K & 1;
-        double ax = px / mass, ay = py / mass, az = pz / mass;
+        const int slot = slot_of(K);
+        double ax, ay, az;
+        if constexpr (MODE == MODE_FUSED) {
+            // one division and three products (not bitwise the quotients)
+            const double rm = 1.0 / mass;
+            ax = px * rm;
+            ay = py * rm;
+            az = pz * rm;
+        } else {
+            ax = px / mass;
+            ay = py / mass;
+            az = pz / mass;
+        }
         if (STORE_F) {
             fo.fx[g] = px;
             fo.fy[g] = py;
@@ -203,24 +248,24 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
             fo.ydd[g] = ay;
             fo.zdd[g] = az;
         }
-        double vx = pn[slot][3][own] + ax * dt;
+        double vx = pnv(slot, 3, own) + ax * dt;
         if (fabs_(vx) < u_cut) vx = 0.0;
-        double vy = pn[slot][4][own] + ay * dt;
+        double vy = pnv(slot, 4, own) + ay * dt;
         if (fabs_(vy) < u_cut) vy = 0.0;
-        double vz = pn[slot][5][own] + az * dt;
+        double vz = pnv(slot, 5, own) + az * dt;
         if (fabs_(vz) < u_cut) vz = 0.0;
         out.xd[g] = vx;
         out.yd[g] = vy;
         out.zd[g] = vz;
-        out.x[g] = pn[slot][0][own] + vx * dt;
-        out.y[g] = pn[slot][1][own] + vy * dt;
-        out.z[g] = pn[slot][2][own] + vz * dt;
+        out.x[g] = pnv(slot, 0, own) + vx * dt;
+        out.y[g] = pnv(slot, 1, own) + vy * dt;
+        out.z[g] = pnv(slot, 2, own) + vz * dt;
     };
-    // Node plane K of the patch goes into ring slot K & 1 in two steps:
-    // fetch_plane issues the global loads into registers (coalesced rows of
-    // 17 nodes; nodes outside the mesh are not touched -- no element reads
-    // them) and store_plane writes them to LDS one phase later, so the load
-    // latency is hidden behind the node work of the current plane.
+    // Ring fill: node plane K goes into its slot in two steps: fetch_plane
+    // issues the global loads into registers (coalesced rows of pw+1 nodes;
+    // nodes outside the mesh are not touched -- no element reads them) and
+    // store_plane writes them to LDS one phase later, so the load latency is
+    // hidden behind the node work of the current plane.
     // Patch node t = tid + 256 m, m = 0, 1.
     int pg[2];
     bool pin[2];
@@ -253,13 +298,13 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
         }
     };
     auto store_plane = [&](int K) {
-        const int slot = K & 1;
+        const int slot = slot_of(K);
 #pragma unroll
         for (int m = 0; m < 2; ++m)
             if (pin[m]) {
                 const int t = tid + m * kZmarchThreads;
 #pragma unroll
-                for (int f = 0; f < 6; ++f) pn[slot][f][t] = rn[m][f];
+                for (int f = 0; f < 6; ++f) pnv(slot, f, t) = rn[m][f];
             }
     };
     // The element scalars needed first are fetched early as well:
@@ -277,14 +322,21 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
             for (int f = 0; f < 4; ++f) ev[f] = 0.0;
         }
     };
-    // add stress then hourglass of one corner staged at lds[4c.. ] of patch thread t
+    // add the corner staged at slot c of patch thread t: stress then
+    // hourglass, or their sum (COMBINE)
     auto add = [&](int c, int t) {
-        px += lds[6 * c + 0][t];
-        px += lds[6 * c + 3][t];
-        py += lds[6 * c + 1][t];
-        py += lds[6 * c + 4][t];
-        pz += lds[6 * c + 2][t];
-        pz += lds[6 * c + 5][t];
+        if constexpr (COMBINE) {
+            px += lds[3 * c + 0][t];
+            py += lds[3 * c + 1][t];
+            pz += lds[3 * c + 2][t];
+        } else {
+            px += lds[6 * c + 0][t];
+            px += lds[6 * c + 3][t];
+            py += lds[6 * c + 1][t];
+            py += lds[6 * c + 4][t];
+            pz += lds[6 * c + 2][t];
+            pz += lds[6 * c + 5][t];
+        }
     };
 
     const int kbeg = max(K0 - 1, 0);
@@ -300,108 +352,130 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
     };
     double mcur = fetch_mass(kbeg);
     for (int k = kbeg; k < kend; ++k) {
-        // plane k is in its slot; plane k+1 (fetched during the previous
+        // Plane k is in its slot; plane k+1 (fetched during the previous
         // step) goes over plane k-1, whose last readers (finish_node) passed
         // the previous step's third barrier.  The previous step's last phase
         // only read the corner staging area, which is written again after
         // this barrier.
         store_plane(k + 1);
         __syncthreads();
-        // Wave priorities (s_setprio) for the two workgroups sharing each
-        // SIMD: the element work issues ahead of the other workgroup's node
-        // phases (its LDS-bound corner sums and node updates fill the issue
-        // slots the FP64 chains leave), the hourglass part highest.
-        // Measured at s=150: nodal 0.301 -> 0.261 ms (separate rounding),
-        // 0.243 -> 0.219 ms (fused); no priority order tried was faster.
+        // Wave priorities (s_setprio) for the workgroups sharing each SIMD:
+        // the element work issues ahead of the other workgroup's node phases
+        // (its LDS-bound corner sums and node updates fill the issue slots
+        // the FP64 chains leave), the hourglass part highest.  Measured at
+        // s=150: nodal 0.301 -> 0.261 ms (separate rounding), 0.243 -> 0.219
+        // ms (fused); no priority order tried was faster.
         __builtin_amdgcn_s_setprio(2);
         // ---- element (k, ej, ei): stress + hourglass corner forces ----------
-        double up[24]; // corners 4..7: (sx, sy, sz, hx, hy, hz) each
+        double up[kStage]; // corners 4..7: (sx, sy, sz, hx, hy, hz) each, or their sums
         if (elemCol) {
             // sigxx = sigyy = sigzz = -p - q (lulesh.cc:629-633) and the
             // hourglass determinant volo*v (lulesh.cc:1098)
             const int e = (k * s + ej) * s + ei;
             const double sig = -ev[0] - ev[1];
             const double determ = ev[2] * ev[3];
+            const double sse = ss[e];
+            const double emass = elemMass[e];
             // the element's nodes in the patch node arrays (reference local
             // order, lulesh-init.cc:237-244): bottom plane k, top plane k+1
             const int q0 = jj * (pw + 1) + ii;
             const int nq[4] = {q0, q0 + 1, q0 + pw + 2, q0 + pw + 1};
-            const int sb = k & 1, st1 = sb ^ 1;
+            const int sb = slot_of(k), st1 = slot_of(k + 1);
+            auto node = [&](int f, int a) { return pnv(a < 4 ? sb : st1, f, nq[a & 3]); };
             double x[8], y[8], z[8];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                x[a] = pn[sb][0][nq[a]];
-                y[a] = pn[sb][1][nq[a]];
-                z[a] = pn[sb][2][nq[a]];
-                x[a + 4] = pn[st1][0][nq[a]];
-                y[a + 4] = pn[st1][1][nq[a]];
-                z[a + 4] = pn[st1][2][nq[a]];
+            for (int a = 0; a < 8; ++a) {
+                x[a] = node(0, a);
+                y[a] = node(1, a);
+                z[a] = node(2, a);
             }
             {
                 double sc[3][8];
                 const double det = shape_derivatives<false>(x, y, z, nullptr);
                 if (det == 0) st->error = kVolumeError;
                 node_normals(sc, x, y, z);
-
                 // corners 0..3 go to LDS now (its last readers passed the
                 // previous barrier); corners 4..7 wait in registers
+                constexpr int W = COMBINE ? 3 : 6; // staged values per corner
 #pragma unroll
                 for (int a = 0; a < 4; ++a)
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
-                        lds[6 * a + d][tid] = -(sig * sc[d][a]);
-                        up[6 * a + d] = -(sig * sc[d][a + 4]);
+                        lds[W * a + d][tid] = -(sig * sc[d][a]);
+                        up[W * a + d] = -(sig * sc[d][a + 4]);
                     }
             }
             if (determ <= 0.0) st->error = kVolumeError;
             __builtin_amdgcn_s_setprio(3);
-            double xd[8], yd[8], zd[8];
+            if constexpr (MODE == MODE_MIXED) {
+                // FP32 hourglass force added to each corner's stress in place
+                // (lower corners in LDS, upper in registers), in two
+                // register-lean passes, on coordinates relative to node 0
+                // (their differences keep their significant bits)
+                float dvdx[8], dvdy[8], dvdz[8], hmx[4], hmy[4], hmz[4];
+                {
+                    float xf[8], yf[8], zf[8];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                xd[a] = pn[sb][3][nq[a]];
-                yd[a] = pn[sb][4][nq[a]];
-                zd[a] = pn[sb][5][nq[a]];
-                xd[a + 4] = pn[st1][3][nq[a]];
-                yd[a + 4] = pn[st1][4][nq[a]];
-                zd[a + 4] = pn[st1][5][nq[a]];
-            }
-            double hg[3][8];
-            if constexpr (sizeof(HG) == 8) {
-                const double coefficient = -hgcoef * 0.01 * ss[e] * elemMass[e] / cbrt_(determ);
-                hourglass_force<double>(x, y, z, xd, yd, zd, determ, coefficient, hg);
-            } else {
-                float xf[8], yf[8], zf[8], xdf[8], ydf[8], zdf[8], hf[3][8];
-#pragma unroll
-                for (int a = 0; a < 8; ++a) {
-                    xf[a] = float(x[a] - x[0]);
-                    yf[a] = float(y[a] - y[0]);
-                    zf[a] = float(z[a] - z[0]);
-                    xdf[a] = float(xd[a]);
-                    ydf[a] = float(yd[a]);
-                    zdf[a] = float(zd[a]);
+                    for (int a = 0; a < 8; ++a) {
+                        xf[a] = float(x[a] - x[0]);
+                        yf[a] = float(y[a] - y[0]);
+                        zf[a] = float(z[a] - z[0]);
+                    }
+                    volume_derivative<float>(dvdx, dvdy, dvdz, xf, yf, zf);
+                    hourglass_modes<float>(xf, yf, zf, hmx, hmy, hmz);
                 }
                 const float df = float(determ);
                 const float coefficient =
-                    float(-hgcoef * 0.01) * float(ss[e]) * float(elemMass[e]) / cbrt_(df);
-                hourglass_force<float>(xf, yf, zf, xdf, ydf, zdf, df, coefficient, hf);
+                    float(-hgcoef * 0.01) * float(sse) * float(emass) / cbrt_(df);
+                fb_hourglass_two_pass<float>(
+                    dvdx, dvdy, dvdz, hmx, hmy, hmz, 1.0f / df, coefficient,
+                    [&](int c, int a) { return float(node(3 + c, a)); },
+                    [&](int c, int a, float f) {
+                        if (a < 4)
+                            lds[3 * a + c][tid] += double(f);
+                        else
+                            up[3 * (a - 4) + c] += double(f);
+                    });
+            } else if constexpr (MODE == MODE_FUSED) {
+                // FP64 hourglass force added to each corner's stress in place,
+                // in two register-lean passes (velocities read from LDS when
+                // used; the sums are the one-pass form's)
+                double dvdx[8], dvdy[8], dvdz[8], hmx[4], hmy[4], hmz[4];
+                volume_derivative<double>(dvdx, dvdy, dvdz, x, y, z);
+                hourglass_modes<double>(x, y, z, hmx, hmy, hmz);
+                const double coefficient = -hgcoef * 0.01 * sse * emass / cbrt_(determ);
+                fb_hourglass_two_pass<double>(
+                    dvdx, dvdy, dvdz, hmx, hmy, hmz, 1.0 / determ, coefficient,
+                    [&](int c, int a) { return node(3 + c, a); },
+                    [&](int c, int a, double f) {
+                        if (a < 4)
+                            lds[3 * a + c][tid] += f;
+                        else
+                            up[3 * (a - 4) + c] += f;
+                    });
+            } else {
+                double xd[8], yd[8], zd[8];
 #pragma unroll
                 for (int a = 0; a < 8; ++a) {
-                    hg[0][a] = double(hf[0][a]);
-                    hg[1][a] = double(hf[1][a]);
-                    hg[2][a] = double(hf[2][a]);
+                    xd[a] = node(3, a);
+                    yd[a] = node(4, a);
+                    zd[a] = node(5, a);
                 }
-            }
+                double hg[3][8];
+                const double coefficient = -hgcoef * 0.01 * sse * emass / cbrt_(determ);
+                hourglass_force<double>(x, y, z, xd, yd, zd, determ, coefficient, hg);
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
+                for (int c = 0; c < 4; ++c) {
 #pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    lds[6 * c + 3 + d][tid] = hg[d][c];
-                    up[6 * c + 3 + d] = hg[d][c + 4];
+                    for (int d = 0; d < 3; ++d) {
+                        lds[6 * c + 3 + d][tid] = hg[d][c];
+                        up[6 * c + 3 + d] = hg[d][c + 4];
+                    }
                 }
             }
         } else {
 #pragma unroll
-            for (int c = 0; c < 24; ++c) up[c] = 0.0;
+            for (int c = 0; c < kStage; ++c) up[c] = 0.0;
         }
         __builtin_amdgcn_sched_barrier(0); // keep the prefetch out of the element work
         // the next step's plane, element scalars and nodal mass
@@ -421,7 +495,7 @@ __global__ __launch_bounds__(kZmarchThreads) __attribute__((amdgpu_waves_per_eu(
         __syncthreads();
         if (elemCol) {
 #pragma unroll
-            for (int c = 0; c < 24; ++c) lds[c][tid] = up[c];
+            for (int c = 0; c < kStage; ++c) lds[c][tid] = up[c];
         }
         __syncthreads();
         // ---- node plane k+1 starts with corners 6,7,5,4 -------------------------
