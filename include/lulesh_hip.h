@@ -165,6 +165,24 @@ int lulesh_hip_run(lulesh_ctx *ctx, int32_t max_cycles, lulesh_time_state *inout
 int lulesh_hip_phase(lulesh_ctx *ctx, int phase, double deltatime);
 int lulesh_hip_constraints_result(lulesh_ctx *ctx, double *dtcourant, double *dthydro);
 
+/* The constraints phase without a context and without any allocation: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid it).  Computes the Courant and hydro minima (CalcCourantConstraint /
+ * CalcHydroConstraint, lulesh.cc:1938-1974; every operation rounded on its
+ * own) over caller-owned device arrays ss, vdov, arealg of numElem elements
+ * and writes out[0] = dtcourant, out[1] = dthydro (device memory, 1e20 where
+ * no element constrains), with the run constants qqc and dvovmax
+ * (lulesh-init.cc:27, 31).  Everything is enqueued on `stream` (a
+ * hipStream_t; NULL = the default stream) and nothing waits for it.  The
+ * partial minima live in a fixed 32 KB device array of the library (one per
+ * device, kept until the process ends), cleared at the start of each call:
+ * calls that overlap on one device race.  0 <= numElem <= 2^31 - 257, else
+ * LULESH_INVALID_ARGUMENT. */
+int lulesh_hip_constraints_noalloc(const double *ss, const double *vdov, const double *arealg,
+                                   int32_t numElem, double qqc, double dvovmax, double *out,
+                                   void *stream);
+
 /* Synchronous copies between HBM and host arrays of `count` elements. */
 int lulesh_hip_download(lulesh_ctx *ctx, int field_id, void *dst, size_t count);
 int lulesh_hip_upload(lulesh_ctx *ctx, int field_id, const void *src, size_t count);

@@ -41,7 +41,9 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstdio>
+#include <climits>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <type_traits>
@@ -96,6 +98,9 @@ inline unsigned long long host_dbits(double v) {
 // positive doubles, whose bit patterns order like the values.
 constexpr int kMinSlots = 256;
 constexpr int kSlotU64 = 16;
+// largest element count of lulesh_hip_constraints_noalloc (int32 indices and
+// a grid of 256-thread workgroups)
+constexpr int32_t kNoAllocMaxElems = INT32_MAX - 256;
 constexpr unsigned long long kMinInit = 0x4415AF1D78B58C40ull; // 1.0e+20
 
 // Minimum over the kMinSlots slots of one problem, by a workgroup of
@@ -777,6 +782,26 @@ __global__ __launch_bounds__(kBlock) void k_constraints(int32_t E, const double 
     double dtc = 1.7976931348623157e+308, dth = 1.7976931348623157e+308;
     if (i < E) constraints(ss[i], vdov[i], arealg[i], P, dtc, dth);
     block_min_atomic(dtc, dth, minslots);
+}
+
+// Arm every constraint-minima slot of one problem at 1e20 (a fill kernel,
+// so that the clearing can sit in a captured graph).
+__global__ __launch_bounds__(kMinSlots) void k_arm_slots(unsigned long long *slots) {
+    unsigned long long *sl = slots + size_t(threadIdx.x) * kSlotU64;
+    sl[0] = kMinInit;
+    sl[1] = kMinInit;
+}
+
+// Fold one problem's slots into out[0] = dtcourant, out[1] = dthydro (and
+// re-arm them); one workgroup of kMinSlots threads.
+__global__ __launch_bounds__(kMinSlots) void k_fold_minima_to(unsigned long long *slots,
+                                                              double *out) {
+    unsigned long long c, h;
+    fold_slots(slots, c, h);
+    if (threadIdx.x == 0) {
+        out[0] = bits_to_double(c);
+        out[1] = bits_to_double(h);
+    }
 }
 
 template <typename T>
@@ -2261,6 +2286,64 @@ int lulesh_hip_phase(lulesh_ctx *c, int phase, double deltatime) {
     if ((rc = read_states(c))) return rc;
     return c->hst->error;
 }
+
+#if !LULESH_FUSED_FORM
+// ---------------------------------------------------------------------------
+// Constraints without allocation (lulesh_hip_constraints_noalloc): the
+// kernels of the constraints phase (k_constraints into per-slot atomic
+// minima, then the slot fold) with the slots in this file-scope array
+// instead of a context's allocation.  Each device has its own copy; its
+// address is looked up once per device.  Calls on one device share it:
+// calls that overlap on one device (different streams, or host threads)
+// race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned long long scratch[kMinSlots * kSlotU64];
+
+namespace {
+constexpr int kMaxDevices = 64;
+unsigned long long *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// The device address of `scratch` on the current device (looked up once).
+hipError_t scratch_on_device(unsigned long long **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<unsigned long long *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+} // namespace
+
+int lulesh_hip_constraints_noalloc(const double *ss, const double *vdov, const double *arealg,
+                                   int32_t numElem, double qqc, double dvovmax, double *out,
+                                   void *stream) {
+    if (!out || numElem < 0 || numElem > kNoAllocMaxElems ||
+        (numElem > 0 && (!ss || !vdov || !arealg)))
+        return LULESH_INVALID_ARGUMENT;
+    unsigned long long *slots = nullptr;
+    hipError_t e = scratch_on_device(&slots);
+    if (e != hipSuccess) return LULESH_HIP_ERROR - int(e);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    Params P{};
+    P.qqc = qqc;
+    P.dvovmax = dvovmax;
+    k_arm_slots<<<1, kMinSlots, 0, st>>>(slots);
+    if (numElem > 0)
+        k_constraints<<<grid_for(numElem, kBlock), kBlock, 0, st>>>(numElem, ss, vdov, arealg, P,
+                                                                    slots);
+    k_fold_minima_to<<<1, kMinSlots, 0, st>>>(slots, out);
+    e = hipGetLastError();
+    return e == hipSuccess ? 0 : LULESH_HIP_ERROR - int(e);
+}
+#endif
 
 int lulesh_hip_constraints_result(lulesh_ctx *c, double *dtcourant, double *dthydro) {
     if (!c) return LULESH_INVALID_ARGUMENT;
