@@ -135,6 +135,31 @@ class Domain {
     Index_t &numElem() { return m_numElem; }
     Index_t &numNode() { return m_numNode; }
 
+    // Raw array pointers, lulesh.h:357-392
+    Real_t *x_begin() { return m_x.data(); }
+    Real_t *y_begin() { return m_y.data(); }
+    Real_t *z_begin() { return m_z.data(); }
+    Real_t *xd_begin() { return m_xd.data(); }
+    Real_t *yd_begin() { return m_yd.data(); }
+    Real_t *zd_begin() { return m_zd.data(); }
+    Real_t *xdd_begin() { return m_xdd.data(); }
+    Real_t *ydd_begin() { return m_ydd.data(); }
+    Real_t *zdd_begin() { return m_zdd.data(); }
+    Real_t *fx_begin() { return m_fx.data(); }
+    Real_t *fy_begin() { return m_fy.data(); }
+    Real_t *fz_begin() { return m_fz.data(); }
+    Index_t *symmX_begin() { return m_symmX.data(); }
+    Index_t *symmY_begin() { return m_symmY.data(); }
+    Index_t *symmZ_begin() { return m_symmZ.data(); }
+    Real_t *p_begin() { return m_p.data(); }
+    Real_t *q_begin() { return m_q.data(); }
+    Real_t *v_begin() { return m_v.data(); }
+    Real_t *vnew_begin() { return m_vnew.data(); }
+    Real_t *nodalMass_begin() { return m_nodalMass.data(); }
+    Real_t *vdov_begin() { return m_vdov.data(); }
+    Real_t *ss_begin() { return m_ss.data(); }
+    Real_t *elemMass_begin() { return m_elemMass.data(); }
+
     // A plain-pointer view of every array for the C-ABI (lulesh_hip.h).
     lulesh_domain_view view();
 
