@@ -67,6 +67,11 @@ def kernel_bytes(name, E, N, mixed=False, material_mode=0):
         # region lists: list, elemBC, 6 gradients + 6 neighbour gradients
         # (gathered), vdov elemMass volo vnew e delv p q arealg in; qq ql p e q ss out
         return E * (4 + 4 + 12 * gr + 72 + 48)
+    if name == "elements_material":
+        # the fused pass: node x..zd in; volo v in; v vnew delv arealg vdov out;
+        # repetition count (u16), elemBC, elemMass e p q in; qq ql p e q ss out
+        # (the Q gradients stay on chip)
+        return E * (16 + 40 + 2 + 4 + 32 + 48) + N * 48
     if name == "time_increment":
         return 128
     return 0
@@ -191,12 +196,12 @@ def copy_bandwidth(pyl, nbytes=1 << 31, reps=10):
     return gbs.value if rc == 0 and gbs.value > 0 else None
 
 
-def measured_traffic_per_cycle(arith):
+def measured_traffic_per_cycle(arith, names):
     """HBM bytes one s=150 FP64 cycle moves by the counters of the committed
-    rocprofv3 summary (FETCH_SIZE + WRITE_SIZE of each kernel, calibrated),
-    or None."""
+    rocprofv3 summary (FETCH_SIZE + WRITE_SIZE of each kernel in `names`,
+    calibrated), or None."""
     tot = 0.0
-    for k in ("nodal", "elements", "material"):
+    for k in names:
         t = measured("traffic", k, arith)
         if t is None:
             return None
@@ -434,7 +439,8 @@ def main():
 
     copy_gbs = None if args.no_copy_reference else copy_bandwidth(pyl)
     bsz = model_bytes_per_zone_cycle(s)
-    traffic_cycle = measured_traffic_per_cycle(args.arith) if prof else None
+    big = [k for k in kernels if k not in ("time_increment", "minima")]
+    traffic_cycle = measured_traffic_per_cycle(args.arith, big) if prof else None
     out = {
         "metric": "zone-cycles/sec FP64 s=150 vs FP32-mixed & 64-problem batch; % HBM peak",
         "value": zcps,

@@ -230,14 +230,23 @@ typedef struct lulesh_tuning {
                                  for a batch) */
     int32_t cycles_per_graph; /* cycles captured per HIP graph (0 = by problem size) */
     int32_t material_mode;    /* 0: element order, EOS repetitions balanced over each wave;
-                                 1: region lists grouped by repetition count */
+                                 1: region lists grouped by repetition count;
+                                 2: fused with the elements phase in one z-march pass
+                                 (structured cube only; the cycle then does not store
+                                 the Q gradients delv_* / delx_*, whose downloads are
+                                 refused until an elements phase stores them again;
+                                 slower than 0 on the Sedov problem: 0.42 against
+                                 0.28 ms per cycle at s = 150);
+                                 other values: 0, or 1 without repetition counts */
+    int32_t em_planes;        /* element planes per workgroup of the fused pass
+                                 (0 = sized to fill the GPU) */
 } lulesh_tuning;
 
 int lulesh_hip_set_tuning(lulesh_ctx *ctx, const lulesh_tuning *tuning);
 int lulesh_hip_get_tuning(const lulesh_ctx *ctx, lulesh_tuning *tuning);
 /* Autotuner: times `cycles` cycles for each candidate launch configuration
  * (coordinate descent over zplanes, block_elem, block_material,
- * cycles_per_graph, material_mode; each candidate is run twice from the
+ * cycles_per_graph, material_mode, em_planes; each candidate is run twice from the
  * snapshot, the second run timed) starting from a snapshot of the current state, restores
  * the state, and keeps the fastest.  Returns it and its ms per cycle.  A
  * stream group (LULESH_OPT_STREAMS) is tuned as a whole: every candidate is

@@ -98,9 +98,6 @@ inline unsigned long long host_dbits(double v) {
 // positive doubles, whose bit patterns order like the values.
 constexpr int kMinSlots = 256;
 constexpr int kSlotU64 = 16;
-// largest element count of lulesh_hip_constraints_noalloc (int32 indices and
-// a grid of 256-thread workgroups)
-constexpr int32_t kNoAllocMaxElems = INT32_MAX - 256;
 constexpr unsigned long long kMinInit = 0x4415AF1D78B58C40ull; // 1.0e+20
 
 // Minimum over the kMinSlots slots of one problem, by a workgroup of
@@ -396,6 +393,32 @@ __global__ __launch_bounds__(kBlock) void k_nodes(
 // ran since (a phase called on its own).
 // NT: the results are stored with the non-temporal hint (stream_put).
 // ---------------------------------------------------------------------------
+// Q gradients of one element in the form's gradient type: FP64, or FP32 on
+// coordinates taken relative to node 0 (their differences keep their
+// significant bits) in the mixed form.  vol = volo * (V / volo), the
+// reference's volo[i] * vnew[i] (lulesh.cc:1349).
+template <typename QG>
+__device__ __forceinline__ void element_q_gradients(const double x[8], const double y[8],
+                                                    const double z[8], const double xd[8],
+                                                    const double yd[8], const double zd[8],
+                                                    double vol, QG g[6]) {
+    if constexpr (sizeof(QG) == 8) {
+        q_gradients<double>(x, y, z, xd, yd, zd, vol, g);
+    } else {
+        float xf[8], yf[8], zf[8], xv[8], yv[8], zv[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            xf[k] = float(x[k] - x[0]);
+            yf[k] = float(y[k] - y[0]);
+            zf[k] = float(z[k] - z[0]);
+            xv[k] = float(xd[k]);
+            yv[k] = float(yd[k]);
+            zv[k] = float(zd[k]);
+        }
+        q_gradients<float>(xf, yf, zf, xv, yv, zv, float(vol), g);
+    }
+}
+
 struct ElemKin {
     const double *volo;
     double *v, *vnew, *delv, *arealg, *vdov;
@@ -449,21 +472,7 @@ __global__ __launch_bounds__(kBlock) void k_elems(
     // Q gradients on the end-of-step geometry (lulesh.cc:1349)
     {
         QG g[6];
-        if constexpr (sizeof(QG) == 8) {
-            q_gradients<double>(x, y, z, xd, yd, zd, voloi * rel, g);
-        } else {
-            float xf[8], yf[8], zf[8], xv[8], yv[8], zv[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                xf[k] = float(x[k] - x[0]);
-                yf[k] = float(y[k] - y[0]);
-                zf[k] = float(z[k] - z[0]);
-                xv[k] = float(xd[k]);
-                yv[k] = float(yd[k]);
-                zv[k] = float(zd[k]);
-            }
-            q_gradients<float>(xf, yf, zf, xv, yv, zv, float(voloi * rel), g);
-        }
+        element_q_gradients<QG>(x, y, z, xd, yd, zd, voloi * rel, g);
         stream_put<NT>(&dxxi[i], g[0]);
         stream_put<NT>(&dxeta[i], g[1]);
         stream_put<NT>(&dxzeta[i], g[2]);
@@ -612,6 +621,84 @@ __global__ __launch_bounds__(kBlock) void k_material(
     block_min_atomic(dtc, dth, minslots + size_t(sg.prob) * kMinSlots * kSlotU64);
 }
 
+constexpr int kMapWords = 16; // rounds of 64 evaluations per start map
+constexpr int kMatSlots = 11; // LDS doubles per element: 7 EOS inputs, 4 outputs
+
+// EvalEOSAllInOneTask x rep (lulesh.cc:2190-2194) of the 64 elements of a
+// wave, dealt out over its lanes.  mat_lds holds kMatSlots doubles per
+// thread of the workgroup: the 7 EOS inputs of its element at [0..6] on
+// entry, the outputs e, p, q, ss of the element's last repetition at
+// [7..10] on return.  smap: kMapWords start-map words per wave; sincl: one
+// int per thread.  Every lane of the wave calls it (rep = 0 for lanes
+// without an element).  The k-th start of the map belongs to the k-th lane
+// with rep > 0: lane k itself when such lanes come first in the wave
+// (COMPACT false), else looked up in lanes (one int per thread, COMPACT).
+template <bool RHO1, bool COMPACT>
+__device__ __forceinline__ void eos_wave_balanced(double *mat_lds, unsigned long long *smap,
+                                                  int32_t *sincl, int32_t *lanes, int tid,
+                                                  int rep, const Params &P) {
+    const int lane = tid & 63, wbase = tid & ~63;
+    if constexpr (COMPACT) {
+        // lanes[wbase + k] = the lane of the wave's k-th element with rep > 0
+        const unsigned long long act = __ballot(rep > 0);
+        const int rank = __builtin_amdgcn_mbcnt_hi(uint32_t(act >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo(uint32_t(act), 0));
+        if (rep > 0) lanes[wbase + rank] = lane;
+    }
+    // exclusive prefix sum of the repetition counts over the wave
+    int incl = rep;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+    }
+    const int total = __builtin_amdgcn_readlane(incl, 63);
+    const int excl = incl - rep;
+    sincl[tid] = incl;
+    // Evaluation w of the wave belongs to the element whose range
+    // [excl, excl + rep) holds it.  Bit b of word k of the wave's start map
+    // marks an element whose first evaluation is k*64 + b, so the owner of
+    // evaluation r*64 + L is (starts before word r) + (starts in word r at
+    // bits <= L) - 1.  The map covers kMapWords rounds at a time.
+    unsigned long long *wmap = smap + (tid >> 6) * kMapWords;
+    int started = 0; // elements whose first evaluation precedes the current round
+    for (int c0 = 0; c0 < total; c0 += 64 * kMapWords) {
+        if (lane < kMapWords) wmap[lane] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (rep > 0 && excl >= c0 && excl < c0 + 64 * kMapWords)
+            atomicOr(&wmap[(excl - c0) >> 6], 1ull << (excl & 63));
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const int nev = min(total - c0, 64 * kMapWords);
+        for (int r = 0; r * 64 < nev; ++r) {
+            const unsigned long long m = wmap[r];
+            const uint32_t mlo = __builtin_amdgcn_readfirstlane(uint32_t(m));
+            const uint32_t mhi = __builtin_amdgcn_readfirstlane(uint32_t(m >> 32));
+            const int below = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0));
+            const int self = int((lane < 32 ? mlo >> lane : mhi >> (lane - 32)) & 1u);
+            if (r * 64 + lane < nev) {
+                const int k = started + below + self - 1; // the owner's rank
+                const int j = wbase + (COMPACT ? lanes[wbase + k] : k);
+                double en, pn, qn, ssn;
+                double *el = mat_lds + kMatSlots * j;
+                eos<RHO1>(el[0], el[1], el[2], el[3], el[4], el[5], el[6], P, en, pn, qn, ssn);
+                if (c0 + r * 64 + lane == sincl[j] - 1) { // the element's last repetition
+                    el[7] = en;
+                    el[8] = pn;
+                    el[9] = qn;
+                    el[10] = ssn;
+                }
+            }
+            started += __builtin_popcount(mlo) + __builtin_popcount(mhi);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // ---------------------------------------------------------------------------
 // Material in element order.  Same per-element work as k_material, but the
 // `rep` evaluations of EvalEOSAllInOneTask of the 64 elements of a wave are
@@ -624,9 +711,6 @@ __global__ __launch_bounds__(kBlock) void k_material(
 // read in order (coalesced) with one round trip to memory.  NT: the
 // results are stored with the non-temporal hint (stream_put).
 // ---------------------------------------------------------------------------
-constexpr int kMapWords = 16; // rounds of 64 evaluations per start map
-constexpr int kMatSlots = 11; // LDS doubles per element: 7 EOS inputs, 4 outputs
-
 template <typename QG, bool RHO1, bool STRUCT, bool NT>
 __global__ __launch_bounds__(kBlock) void k_material_ordered(
     const ProbDesc *__restrict__ pds, const int32_t *__restrict__ blockProb,
@@ -648,7 +732,7 @@ __global__ __launch_bounds__(kBlock) void k_material_ordered(
     DevState *st = states + prob;
     if (!st->active) return; // uniform across the block
     const ProbDesc pd = pds[prob];
-    const int tid = threadIdx.x, lane = tid & 63, wbase = tid & ~63;
+    const int tid = threadIdx.x;
     const int li = (blk - pd.mBlock0) * blockDim.x + tid;
     const bool valid = li < pd.E;
     const int i = pd.elemOff + (valid ? li : pd.E - 1);
@@ -707,57 +791,7 @@ __global__ __launch_bounds__(kBlock) void k_material_ordered(
         me[5] = qlin;
         me[6] = vnewc;
     }
-    // exclusive prefix sum of the repetition counts over the wave
-    int incl = rep;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int v = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += v;
-    }
-    const int total = __builtin_amdgcn_readlane(incl, 63);
-    const int excl = incl - rep;
-    sincl[tid] = incl;
-    // Evaluation w of the wave belongs to the element whose range
-    // [excl, excl + rep) holds it.  Bit b of word k of the wave's start map
-    // marks an element whose first evaluation is k*64 + b, so the owner of
-    // evaluation r*64 + L is (starts before word r) + (starts in word r at
-    // bits <= L) - 1.  The map covers kMapWords rounds at a time.
-    unsigned long long *wmap = smap + (tid >> 6) * kMapWords;
-    int started = 0; // elements whose first evaluation precedes the current round
-    for (int c0 = 0; c0 < total; c0 += 64 * kMapWords) {
-        if (lane < kMapWords) wmap[lane] = 0;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        if (rep > 0 && excl >= c0 && excl < c0 + 64 * kMapWords)
-            atomicOr(&wmap[(excl - c0) >> 6], 1ull << (excl & 63));
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        const int nev = min(total - c0, 64 * kMapWords);
-        for (int r = 0; r * 64 < nev; ++r) {
-            const unsigned long long m = wmap[r];
-            const uint32_t mlo = __builtin_amdgcn_readfirstlane(uint32_t(m));
-            const uint32_t mhi = __builtin_amdgcn_readfirstlane(uint32_t(m >> 32));
-            const int below = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0));
-            const int self = int((lane < 32 ? mlo >> lane : mhi >> (lane - 32)) & 1u);
-            if (r * 64 + lane < nev) {
-                const int j = wbase + started + below + self - 1;
-                double en, pn, qn, ssn;
-                double *el = mat_lds + kMatSlots * j;
-                eos<RHO1>(el[0], el[1], el[2], el[3], el[4], el[5], el[6], P, en, pn, qn, ssn);
-                if (c0 + r * 64 + lane == sincl[j] - 1) { // the element's last repetition
-                    el[7] = en;
-                    el[8] = pn;
-                    el[9] = qn;
-                    el[10] = ssn;
-                }
-            }
-            started += __builtin_popcount(mlo) + __builtin_popcount(mhi);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    eos_wave_balanced<RHO1, false>(mat_lds, smap, sincl, nullptr, tid, rep, P);
     if (valid) {
         const double *me = mat_lds + kMatSlots * tid;
         const double en = me[7], pn = me[8], qn = me[9], ssn = me[10];
@@ -766,6 +800,258 @@ __global__ __launch_bounds__(kBlock) void k_material_ordered(
         stream_put<NT>(&f.e[i], en);
         stream_put<NT>(&f.q[i], qn);
         constraints(ssn, vdov, arealg, P, dtc, dth);
+    }
+    block_min_atomic(dtc, dth, minslots + size_t(prob) * kMinSlots * kSlotU64);
+}
+
+// ---------------------------------------------------------------------------
+// Elements and material in one pass over the structured cube, without
+// storing the Q gradients: CalcKinematicsForElemsTask, UpdateVolumes and
+// CalcMonotonicQGradientsForElemsTask (lulesh.cc:1169-1418), then
+// CalcMonotonicQRegionForElemsAndApplyInitTask, EvalEOSAllInOneTask x rep,
+// CalcSoundSpeedForElemsAndSaveTask and the element constraints
+// (lulesh.cc:1420-1974), per element the same arithmetic as k_elems and
+// k_material_ordered.
+//
+// A workgroup owns a tile of pw x ph element columns (one thread each) over
+// a range [k0, k1) of element planes and marches upward in z.  Step t
+// computes the kinematics and gradients of plane t, then the material of
+// plane t-1, whose monotonic limiter reads the gradients of its six face
+// neighbours:
+//   zeta (planes t-2 and t)  the same thread's registers;
+//   xi, eta (in the plane)   the tile's gradient grids in LDS, whose border
+//                            holds the elements just outside the tile,
+//                            recomputed by the first wave (2 (pw + ph) <= 64
+//                            of them per plane);
+//   below k0, above k1 - 1   delv_zeta of planes k0-1 and k1 recomputed by
+//                            every thread for its column.
+// A recomputed element repeats its owner's operations on the same inputs,
+// so it has the owner's gradients (bitwise, in the separately rounded
+// form).  The grids rotate over three slots, so one barrier per step
+// orders the writes of plane t after the reads of plane t-2.  The EOS
+// repetitions are dealt out over each wave as in k_material_ordered (idle
+// lanes of a partial tile sit between the elements of a wave: COMPACT).
+// NT: the results are stored with the non-temporal hint (stream_put).
+//
+// Measured at s = 150 (FP64, multiply-adds fused): 0.42 ms per cycle against
+// 0.145 + 0.133 ms for k_elems + k_material_ordered, although it moves about
+// 100 fewer bytes per element.  254 VGPRs hold it to two waves per SIMD
+// (three spill 97 VGPRs: 0.84 ms); its kinematics steps alone take 0.20 ms
+// and its material steps alone 0.167 ms (each slower than its own kernel,
+// which runs at three and seven waves per SIMD), the recomputed border
+// 0.03 ms.  material_mode 2 selects it; the default stays 0.
+// ---------------------------------------------------------------------------
+constexpr int kEmThreads = 256;
+constexpr int kEmGrid = 18 * 18; // (pw + 2)(ph + 2) for pw + ph <= 32
+
+template <typename QG, bool RHO1, bool NT>
+__global__ __launch_bounds__(kEmThreads) void k_elemmat_zmarch(
+    const ProbDesc *__restrict__ pds, const int32_t *__restrict__ blockProb, NodeBufs nb,
+    ElemKin f0, ElemFields f, const uint16_t *__restrict__ erep, Params P, DevState *states,
+    unsigned long long *__restrict__ minslots) {
+    __shared__ double mat_lds[kMatSlots * kEmThreads];
+    __shared__ unsigned long long smap[(kEmThreads / 64) * kMapWords];
+    __shared__ int32_t sincl[kEmThreads], elanes[kEmThreads];
+    __shared__ QG gxi[3][kEmGrid], geta[3][kEmGrid];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int prob = blockProb[blk];
+    DevState *st = states + prob;
+    if (!st->active) return; // uniform across the block
+    const ProbDesc pd = pds[prob];
+    const int s = pd.s, nn = s + 1, pw = pd.emPW, ph = pd.emPH, gw = pw + 2;
+    const int eo = pd.elemOff, no = pd.nodeOff;
+    const int lb = blk - pd.emBlock0, tilesXY = pd.emTX * pd.emTY;
+    const int tile = lb % tilesXY, zg = lb / tilesXY;
+    const int E0x = (tile % pd.emTX) * pw, E0y = (tile / pd.emTX) * ph;
+    const int k0 = zg * pd.emPlanes, k1 = min(k0 + pd.emPlanes, s); // planes [k0, k1)
+    const int tid = threadIdx.x;
+    const int jj = tid / pw, ii = tid - jj * pw;
+    const bool inTile = jj < ph;
+    const int ei = E0x + ii, ej = E0y + jj; // the thread's element column
+    const bool elemCol = inTile && ei < s && ej < s;
+    // grid position of the thread's element (threads past the tile: a
+    // harmless interior slot; their results are not stored)
+    const int own = inTile ? (jj + 1) * gw + ii + 1 : gw + 1;
+    // the border element of lane tid < 2 (pw + ph) of the first wave
+    // (sides xi-, xi+, eta-, eta+), the gradient it contributes and where
+    int hx = -1, hy = -1, hpos = 0;
+    bool hxi = false;
+    {
+        int h = tid;
+        if (h < ph) {
+            hx = E0x - 1, hy = E0y + h, hpos = (h + 1) * gw, hxi = true;
+        } else if ((h -= ph) < ph) {
+            hx = E0x + pw, hy = E0y + h, hpos = (h + 1) * gw + pw + 1, hxi = true;
+        } else if ((h -= ph) < pw) {
+            hx = E0x + h, hy = E0y - 1, hpos = h + 1;
+        } else if ((h -= pw) < pw) {
+            hx = E0x + h, hy = E0y + ph, hpos = (ph + 1) * gw + h + 1;
+        }
+    }
+    const bool border = hx >= 0 && hx < s && hy >= 0 && hy < s;
+    const double deltaTime = st->deltatime;
+    double *const *nf = nb.b[st->cur ^ st->fresh];
+    // element (k, j, i) -> its eight nodes, lulesh-init.cc:236-244
+    auto nodes_of = [&](int k, int j, int i, int n[8]) {
+        const int b = no + (k * nn + j) * nn + i;
+        n[0] = b; n[1] = b + 1; n[2] = b + nn + 1; n[3] = b + nn;
+        n[4] = b + nn * nn; n[5] = b + nn * nn + 1; n[6] = b + nn * nn + nn + 1;
+        n[7] = b + nn * nn + nn;
+    };
+    // the gradients alone (volume, then the Q gradients on it)
+    auto gradients_of = [&](int k, int j, int i, QG g[6]) {
+        int n[8];
+        nodes_of(k, j, i, n);
+        double x[8], y[8], z[8], xd[8], yd[8], zd[8];
+        load8(nf[0], n, x);
+        load8(nf[1], n, y);
+        load8(nf[2], n, z);
+        load8(nf[3], n, xd);
+        load8(nf[4], n, yd);
+        load8(nf[5], n, zd);
+        const double voloi = f0.volo[eo + (k * s + j) * s + i];
+        const double rel = elem_volume<double>(x, y, z) / voloi;
+        element_q_gradients<QG>(x, y, z, xd, yd, zd, voloi * rel, g);
+    };
+    // material inputs of plane t-1 carried from its kinematics
+    double m_vdov = 0.0, m_vnew = 0.0, m_delv = 0.0, m_arealg = 0.0, m_volo = 0.0;
+    QG m_dx[3] = {QG(0), QG(0), QG(0)};
+    QG z_lo = QG(0), z_mid = QG(0); // delv_zeta of planes t-2, t-1
+    double dtc = 1.7976931348623157e+308, dth = 1.7976931348623157e+308;
+    const int tlo = k0 > 0 ? k0 - 1 : 0;
+    for (int t = tlo; t <= k1; ++t) {
+        const int sl = t % 3;
+        QG z_hi = QG(0);
+        double n_vdov = 0.0, n_vnew = 0.0, n_delv = 0.0, n_arealg = 0.0, n_volo = 0.0;
+        QG n_dx[3] = {QG(0), QG(0), QG(0)};
+        if (t >= k0 && t < k1) {
+            // ---- kinematics of plane t (k_elems) -------------------------
+            if (elemCol) {
+                const int i = eo + (t * s + ej) * s + ei;
+                int n[8];
+                nodes_of(t, ej, ei, n);
+                double x[8], y[8], z[8], xd[8], yd[8], zd[8];
+                load8(nf[0], n, x);
+                load8(nf[1], n, y);
+                load8(nf[2], n, z);
+                load8(nf[3], n, xd);
+                load8(nf[4], n, yd);
+                load8(nf[5], n, zd);
+                const double voloi = f0.volo[i];
+                const double vold = f0.v[i];
+                const double volume = elem_volume<double>(x, y, z);
+                const double rel = volume / voloi;
+                const double arealg = characteristic_length(x, y, z, volume);
+                QG g[6];
+                element_q_gradients<QG>(x, y, z, xd, yd, zd, voloi * rel, g);
+                const double dt2 = 0.5 * deltaTime;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    x[k] -= dt2 * xd[k];
+                    y[k] -= dt2 * yd[k];
+                    z[k] -= dt2 * zd[k];
+                }
+                double B[3][8];
+                const double detJ = shape_derivatives<true>(x, y, z, B);
+                const double vdov = velocity_divergence(xd, yd, zd, B, detJ);
+                double vc = vold;
+                if (P.eosvmin != 0.0 && vc < P.eosvmin) vc = P.eosvmin;
+                if (P.eosvmax != 0.0 && vc > P.eosvmax) vc = P.eosvmax;
+                if (vc < 0.0) st->error = kVolumeError;
+                stream_put<NT>(&f0.vnew[i], rel);
+                stream_put<NT>(&f0.delv[i], rel - vold);
+                stream_put<NT>(&f0.arealg[i], arealg);
+                stream_put<NT>(&f0.vdov[i], vdov);
+                stream_put<NT>(&f0.v[i], (fabs_(rel - 1.0) < P.v_cut) ? 1.0 : rel);
+                gxi[sl][own] = g[3];
+                geta[sl][own] = g[4];
+                z_hi = g[5];
+                n_dx[0] = g[0];
+                n_dx[1] = g[1];
+                n_dx[2] = g[2];
+                n_vdov = vdov;
+                n_vnew = rel;
+                n_delv = rel - vold;
+                n_arealg = arealg;
+                n_volo = voloi;
+            }
+            if (border) { // the tile's border element of this lane
+                QG g[6];
+                gradients_of(t, hy, hx, g);
+                (hxi ? gxi[sl] : geta[sl])[hpos] = hxi ? g[3] : g[4];
+            }
+        } else if (t < s && elemCol) { // plane k0 - 1 or k1: delv_zeta alone
+            QG g[6];
+            gradients_of(t, ej, ei, g);
+            z_hi = g[5];
+        }
+        __syncthreads();
+        if (t > k0) {
+            // ---- material of plane m = t - 1 (k_material_ordered) ----------
+            const int m = t - 1, ms = m % 3;
+            const int i = eo + (elemCol ? (m * s + ej) * s + ei : 0);
+            const int32_t bc = f.elemBC[i];
+            const double emass = f.elemMass[i];
+            const double e0 = f.e[i], p0 = f.p[i], q0 = f.q[i];
+            const int rep = elemCol ? int(erep[i]) : 0;
+            const double sxi = double(gxi[ms][own]), seta = double(geta[ms][own]),
+                         szeta = double(z_mid);
+            // BC switch of lulesh.cc:1460-1597 on the neighbours' gradients
+            auto sel = [&](int32_t bits, int32_t comm, int32_t symm, double nbv, double self) {
+                return (bits == comm || bits == 0) ? nbv : (bits == symm ? self : 0.0);
+            };
+            const double phixi =
+                mono_phi(sxi, sel(bc & XI_M, XI_M_COMM, XI_M_SYMM, double(gxi[ms][own - 1]), sxi),
+                         sel(bc & XI_P, XI_P_COMM, XI_P_SYMM, double(gxi[ms][own + 1]), sxi), P);
+            const double phieta = mono_phi(
+                seta, sel(bc & ETA_M, ETA_M_COMM, ETA_M_SYMM, double(geta[ms][own - gw]), seta),
+                sel(bc & ETA_P, ETA_P_COMM, ETA_P_SYMM, double(geta[ms][own + gw]), seta), P);
+            const double phizeta =
+                mono_phi(szeta, sel(bc & ZETA_M, ZETA_M_COMM, ZETA_M_SYMM, double(z_lo), szeta),
+                         sel(bc & ZETA_P, ZETA_P_COMM, ZETA_P_SYMM, double(z_hi), szeta), P);
+            const double dv[3] = {sxi, seta, szeta};
+            const double dx[3] = {double(m_dx[0]), double(m_dx[1]), double(m_dx[2])};
+            double qlin, qquad;
+            q_terms(phixi, phieta, phizeta, m_vdov, dv, dx, emass, m_volo, m_vnew, P, qlin,
+                    qquad);
+            double vnewc = m_vnew;
+            if (P.eosvmin != 0.0 && vnewc < P.eosvmin) vnewc = P.eosvmin;
+            if (P.eosvmax != 0.0 && vnewc > P.eosvmax) vnewc = P.eosvmax;
+            if (elemCol) {
+                stream_put<NT>(&f.qq[i], qquad);
+                stream_put<NT>(&f.ql[i], qlin);
+            }
+            double *me = mat_lds + kMatSlots * tid;
+            me[0] = e0;
+            me[1] = m_delv;
+            me[2] = p0;
+            me[3] = q0;
+            me[4] = qquad;
+            me[5] = qlin;
+            me[6] = vnewc;
+            eos_wave_balanced<RHO1, true>(mat_lds, smap, sincl, elanes, tid, rep, P);
+            if (elemCol) {
+                const double en = me[7], pn = me[8], qn = me[9], ssn = me[10];
+                stream_put<NT>(&f.ss[i], ssn);
+                stream_put<NT>(&f.p[i], pn);
+                stream_put<NT>(&f.e[i], en);
+                stream_put<NT>(&f.q[i], qn);
+                double ec, eh; // this element's constraints, folded into the thread's minima
+                constraints(ssn, m_vdov, m_arealg, P, ec, eh);
+                dtc = (ec < dtc) ? ec : dtc;
+                dth = (eh < dth) ? eh : dth;
+            }
+        }
+        z_lo = z_mid;
+        z_mid = z_hi;
+        m_vdov = n_vdov;
+        m_vnew = n_vnew;
+        m_delv = n_delv;
+        m_arealg = n_arealg;
+        m_volo = n_volo;
+        m_dx[0] = n_dx[0];
+        m_dx[1] = n_dx[1];
+        m_dx[2] = n_dx[2];
     }
     block_min_atomic(dtc, dth, minslots + size_t(prob) * kMinSlots * kSlotU64);
 }
@@ -784,6 +1070,7 @@ __global__ __launch_bounds__(kBlock) void k_constraints(int32_t E, const double 
     block_min_atomic(dtc, dth, minslots);
 }
 
+#if !LULESH_FUSED_FORM
 // Arm every constraint-minima slot of one problem at 1e20 (a fill kernel,
 // so that the clearing can sit in a captured graph).
 __global__ __launch_bounds__(kMinSlots) void k_arm_slots(unsigned long long *slots) {
@@ -803,6 +1090,7 @@ __global__ __launch_bounds__(kMinSlots) void k_fold_minima_to(unsigned long long
         out[1] = bits_to_double(h);
     }
 }
+#endif
 
 template <typename T>
 __global__ void k_widen(int32_t n, const T *__restrict__ src, double *__restrict__ dst) {
@@ -908,6 +1196,11 @@ struct lulesh_ctx {
     int32_t *d_zBlockProb = nullptr, *d_eBlockProb = nullptr, *d_mBlockSeg = nullptr;
     int32_t *d_oBlockProb = nullptr; // element-order material grid
     int oBlocks = 0;
+    int32_t *d_emBlockProb = nullptr; // fused elements + material grid
+    int emBlocks = 0;
+    // the Q gradients in HBM are those of the last elements phase; a cycle
+    // of the fused elements + material pass does not store them
+    bool grads_stale = false;
     uint16_t *erep = nullptr;        // EOS repetitions per element
     unsigned long long *minslots = nullptr; // constraint minima slots, kMinSlots per problem
     // time states
@@ -1035,7 +1328,7 @@ int flush_timers(lulesh_ctx *c) {
     return 0;
 }
 
-enum { T_TIME = 0, T_NODAL, T_NODES, T_ELEMS, T_MATERIAL, T_MINIMA, T_COUNT };
+enum { T_TIME = 0, T_NODAL, T_NODES, T_ELEMS, T_MATERIAL, T_MINIMA, T_ELEMMAT, T_COUNT };
 
 // Node planes per z-march workgroup when the tuning leaves it open: the
 // largest candidate that still gives the launch about eight workgroups per
@@ -1085,6 +1378,44 @@ int auto_zplanes(const lulesh_ctx *c) {
     return std::min(4, nmax);
 }
 
+// Element planes per workgroup of the fused elements + material pass when
+// the tuning leaves it open.  One problem: the count that minimises (rounds
+// of workgroups) x (planes walked, the two recomputed delv_zeta planes
+// counted as one), a round being two workgroups per CU.  A batch: the
+// largest count that still gives 2048 workgroups.
+int auto_em_planes(const lulesh_ctx *c) {
+    auto tiles = [](int s) {
+        int pw, ph;
+        elemmat_shape(s, pw, ph);
+        return int64_t((s + pw - 1) / pw) * ((s + ph - 1) / ph);
+    };
+    if (c->nprob == 1) {
+        const int s = c->probs[0].s;
+        const int64_t t2 = tiles(s), slots = 2 * int64_t(std::max(c->cus, 1));
+        int best = s;
+        int64_t best_cost = INT64_MAX;
+        for (int z = s; z >= 1; --z) { // ties: the larger z (fewer recomputed planes)
+            const int64_t wgs = t2 * ((s + z - 1) / z);
+            const int64_t cost = ((wgs + slots - 1) / slots) * (z + 1);
+            if (cost < best_cost) {
+                best_cost = cost;
+                best = z;
+            }
+        }
+        return best;
+    }
+    int smax = 1;
+    for (const auto &p : c->probs) smax = std::max(smax, p.s);
+    auto blocks = [&](int z) {
+        int64_t b = 0;
+        for (const auto &p : c->probs) b += tiles(p.s) * ((p.s + z - 1) / z);
+        return b;
+    };
+    for (int z : {32, 24, 16, 12, 10, 8, 6, 5, 4})
+        if (z <= smax && blocks(z) >= 2048) return z;
+    return std::min(4, smax);
+}
+
 void destroy_graphs(lulesh_ctx *c) {
     for (hipGraphExec_t g : c->graphs)
         if (g) (void)hipGraphExecDestroy(g);
@@ -1102,9 +1433,17 @@ int configure(lulesh_ctx *c) {
         t.block_material = c->nprob == 1 ? 64 : 128;
     if (t.zplanes <= 0) t.zplanes = auto_zplanes(c);
     if (t.cycles_per_graph <= 0) t.cycles_per_graph = c->totE >= 1000000 ? 16 : 32;
-    if (t.material_mode != 1 || !c->erep) t.material_mode = c->erep ? 0 : 1;
+    // material: 0 (element order) where the per-element repetition counts
+    // exist, else 1; 2 (fused with the elements, structured cube) only when
+    // asked for: at s = 150 it takes 0.42 ms per cycle against 0.28 ms for
+    // the two kernels of mode 0 (k_elemmat_zmarch)
+    const bool can_fuse = c->structured && c->erep;
+    if (t.material_mode < 0 || t.material_mode > 2) t.material_mode = 0;
+    if (t.material_mode == 2 && !can_fuse) t.material_mode = 0;
+    if (t.material_mode == 0 && !c->erep) t.material_mode = 1;
     if (t.material_mode == 1 && !c->matList) t.material_mode = 0; // no region lists
-    std::vector<int32_t> zb, eb, mb, ob;
+    if (t.em_planes <= 0) t.em_planes = auto_em_planes(c);
+    std::vector<int32_t> zb, eb, mb, ob, emb;
     c->hpd.clear();
     c->zSquare16 = true;
     c->streamStores = c->nprob == 1;
@@ -1132,6 +1471,14 @@ int configure(lulesh_ctx *c) {
         eb.insert(eb.end(), size_t(grid_for(ph.E, t.block_elem)), pi);
         d.mBlock0 = int32_t(ob.size());
         ob.insert(ob.end(), size_t(grid_for(ph.E, t.block_material)), pi);
+        elemmat_shape(ph.s, d.emPW, d.emPH);
+        d.emTX = (ph.s + d.emPW - 1) / d.emPW;
+        d.emTY = (ph.s + d.emPH - 1) / d.emPH;
+        d.emPlanes = std::max(1, std::min(t.em_planes, ph.s));
+        d.emBlock0 = int32_t(emb.size());
+        emb.insert(emb.end(),
+                   size_t(d.emTX) * size_t(d.emTY) * size_t((ph.s + d.emPlanes - 1) / d.emPlanes),
+                   pi);
         c->hpd.push_back(d);
     }
     // material segments, the costliest (largest rep) first so they start early
@@ -1150,6 +1497,7 @@ int configure(lulesh_ctx *c) {
     c->eBlocks = int(eb.size());
     c->mBlocks = int(mb.size());
     c->oBlocks = int(ob.size());
+    c->emBlocks = int(emb.size());
     auto put = [&](auto **dptr, const auto &vec) -> int {
         using T = std::remove_reference_t<decltype(vec[0])>;
         if (*dptr) (void)hipFree(*dptr);
@@ -1164,7 +1512,8 @@ int configure(lulesh_ctx *c) {
     int rc;
     if ((rc = put(&c->d_pd, c->hpd)) || (rc = put(&c->d_segs, c->hsegs)) ||
         (rc = put(&c->d_zBlockProb, zb)) || (rc = put(&c->d_eBlockProb, eb)) ||
-        (rc = put(&c->d_mBlockSeg, mb)) || (rc = put(&c->d_oBlockProb, ob)))
+        (rc = put(&c->d_mBlockSeg, mb)) || (rc = put(&c->d_oBlockProb, ob)) ||
+        (rc = put(&c->d_emBlockProb, emb)))
         return rc;
     destroy_graphs(c);
     return 0;
@@ -1242,7 +1591,7 @@ auto material_ordered_kernel(bool rho1, bool structured) {
 
 template <typename R>
 int enqueue_material(lulesh_ctx *c) {
-    if (c->tune.material_mode == 0) {
+    if (c->tune.material_mode != 1) { // (mode 2 outside the fused cycle: element order)
         const ElemFields ef{c->elemBC, c->lxim, c->lxip, c->letam, c->letap, c->lzetam,
                             c->lzetap, c->f[LULESH_F_VDOV], c->f[LULESH_F_ELEMMASS],
                             c->f[LULESH_F_VOLO], c->f[LULESH_F_VNEW], c->f[LULESH_F_DELV],
@@ -1278,6 +1627,28 @@ int enqueue_material(lulesh_ctx *c) {
     });
 }
 
+// The fused elements + material pass (material_mode 2).
+template <typename R>
+int enqueue_elements_material(lulesh_ctx *c) {
+    const ElemKin ek{c->f[LULESH_F_VOLO], c->f[LULESH_F_V], c->f[LULESH_F_VNEW],
+                     c->f[LULESH_F_DELV], c->f[LULESH_F_AREALG], c->f[LULESH_F_VDOV]};
+    const ElemFields ef{c->elemBC, c->lxim, c->lxip, c->letam, c->letap, c->lzetam, c->lzetap,
+                        c->f[LULESH_F_VDOV], c->f[LULESH_F_ELEMMASS], c->f[LULESH_F_VOLO],
+                        c->f[LULESH_F_VNEW], c->f[LULESH_F_DELV], c->f[LULESH_F_AREALG],
+                        c->f[LULESH_F_E], c->f[LULESH_F_P], c->f[LULESH_F_Q], c->f[LULESH_F_QL],
+                        c->f[LULESH_F_QQ], c->f[LULESH_F_SS]};
+    c->grads_stale = true;
+    return launch(c, T_ELEMMAT, [&] {
+        const bool rho1 = c->P.refdens == 1.0;
+        auto kern = c->streamStores
+                        ? (rho1 ? k_elemmat_zmarch<R, true, true> : k_elemmat_zmarch<R, false, true>)
+                        : (rho1 ? k_elemmat_zmarch<R, true, false>
+                                : k_elemmat_zmarch<R, false, false>);
+        kern<<<c->emBlocks, kEmThreads, 0, c->stream>>>(c->d_pd, c->d_emBlockProb, c->nb, ek, ef,
+                                                        c->erep, c->P, c->st, c->minslots);
+    });
+}
+
 // Fold the per-slot constraint minima into each problem's DevState.
 int enqueue_minima(lulesh_ctx *c) {
     return launch(c, T_MINIMA, [&] {
@@ -1295,12 +1666,15 @@ int enqueue_material_and_minima(lulesh_ctx *c) {
 // (pending) until the next k_time_increment or k_reduce_minima folds them.
 int enqueue_cycle_body(lulesh_ctx *c) {
     int rc;
+    const bool fused = c->tune.material_mode == 2;
     if (c->precision == LULESH_PRECISION_MIXED) {
         if ((rc = enqueue_nodal<float>(c))) return rc;
+        if (fused) return enqueue_elements_material<float>(c);
         if ((rc = enqueue_elements<float>(c))) return rc;
         return enqueue_material<float>(c);
     }
     if ((rc = enqueue_nodal<double>(c))) return rc;
+    if (fused) return enqueue_elements_material<double>(c);
     if ((rc = enqueue_elements<double>(c))) return rc;
     return enqueue_material<double>(c);
 }
@@ -1621,7 +1995,7 @@ int create_impl(const lulesh_domain_view *const *views, int nprob, int precision
     c->P = params_of(views[0]);
     c->hgcoef = views[0]->hgcoef;
     c->timers = {{"time_increment"}, {"nodal"}, {"nodes"}, {"elements"}, {"material"},
-                 {"minima"}};
+                 {"minima"}, {"elements_material"}};
     int rc;
     int dev = 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -1816,6 +2190,11 @@ int field_copy(lulesh_ctx *c, int prob, int id, void *host, size_t count, bool d
                  "(LULESH_OPT_NODE_FORCES was not given)";
         return LULESH_INVALID_ARGUMENT;
     }
+    if (down && is_grad_field(id) && c->grads_stale) {
+        c->err = "the Q gradients were not stored by the last cycle (fused elements + "
+                 "material pass); an elements phase stores them";
+        return LULESH_INVALID_ARGUMENT;
+    }
     int rc = read_states(c);
     if (rc) return rc;
     if (c->precision == LULESH_PRECISION_MIXED && is_grad_field(id)) {
@@ -1994,7 +2373,7 @@ int create_sedov_impl(const int32_t *sizes, const int32_t *regions, const double
                   D::m_dvovmax, D::m_refdens};
     c->hgcoef = D::m_hgcoef;
     c->timers = {{"time_increment"}, {"nodal"}, {"nodes"}, {"elements"}, {"material"},
-                 {"minima"}};
+                 {"minima"}, {"elements_material"}};
     int dev = 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->run0) != hipSuccess || hipEventCreate(&c->run1) != hipSuccess ||
@@ -2270,6 +2649,7 @@ int lulesh_hip_phase(lulesh_ctx *c, int phase, double deltatime) {
     case LULESH_PHASE_NODAL: rc = mixed ? enqueue_nodal<float>(c) : enqueue_nodal<double>(c); break;
     case LULESH_PHASE_ELEMENTS:
         rc = mixed ? enqueue_elements<float>(c) : enqueue_elements<double>(c);
+        if (rc == 0) c->grads_stale = false;
         break;
     case LULESH_PHASE_MATERIAL:
         rc = mixed ? enqueue_material_and_minima<float>(c)
@@ -2298,6 +2678,9 @@ int lulesh_hip_phase(lulesh_ctx *c, int phase, double deltatime) {
 // race.
 // ---------------------------------------------------------------------------
 static __device__ __align__(256) unsigned long long scratch[kMinSlots * kSlotU64];
+// largest element count of lulesh_hip_constraints_noalloc (int32 indices and
+// a grid of 256-thread workgroups)
+constexpr int32_t kNoAllocMaxElems = INT32_MAX - 256;
 
 namespace {
 constexpr int kMaxDevices = 64;
@@ -2490,7 +2873,9 @@ int lulesh_hip_download_view(lulesh_ctx *c, lulesh_domain_view *dst) {
     if (!c || !dst || c->nprob != 1) return LULESH_INVALID_ARGUMENT;
     for (int id = 0; id < LULESH_F_COUNT; ++id) {
         double *p = view_field(dst, id);
-        if (!p || (is_force_field(id) && !c->store_forces)) continue;
+        if (!p || (is_force_field(id) && !c->store_forces) ||
+            (is_grad_field(id) && c->grads_stale))
+            continue;
         int rc = lulesh_hip_download(c, id, p, is_node_field(id) ? size_t(c->totN)
                                                                  : size_t(c->totE));
         if (rc) return rc;
@@ -2641,18 +3026,33 @@ int lulesh_hip_autotune(lulesh_ctx *c, int32_t cycles, lulesh_tuning *best,
     }
     const int bcand[] = {64, 128, 256};
     const int gcand[] = {4, 8, 16, 32};
-    const int mcand[] = {0, 1};
-    for (int pass = 0; pass < 5 && rc == 0; ++pass) {
-        const int *cand =
-            pass == 0 ? zcand.data() : pass == 3 ? gcand : pass == 4 ? mcand : bcand;
-        const int ncand = pass == 0 ? int(zcand.size()) : pass == 3 ? 4 : pass == 4 ? 2 : 3;
+    const int mcand[] = {0, 1, 2};
+    // element planes per workgroup of the fused pass: 2..40 groups along z
+    std::vector<int> ecand;
+    for (int g = 40; g >= 2; --g) {
+        const int z = (nmax - 1 + g - 1) / g;
+        if (z >= 2 && (ecand.empty() || ecand.back() != z)) ecand.push_back(z);
+    }
+    for (int pass = 0; pass < 6 && rc == 0; ++pass) {
+        const int *cand = pass == 0   ? zcand.data()
+                          : pass == 3 ? gcand
+                          : pass == 4 ? mcand
+                          : pass == 5 ? ecand.data()
+                                      : bcand;
+        const int ncand = pass == 0   ? int(zcand.size())
+                          : pass == 3 ? 4
+                          : pass == 4 ? 3
+                          : pass == 5 ? int(ecand.size())
+                                      : 3;
+        if (pass == 5 && cur.material_mode != 2) break;
         for (int k = 0; k < ncand && rc == 0; ++k) {
             lulesh_tuning t = cur;
             int32_t *field = pass == 0   ? &t.zplanes
                              : pass == 1 ? &t.block_elem
                              : pass == 2 ? &t.block_material
                              : pass == 3 ? &t.cycles_per_graph
-                                         : &t.material_mode;
+                             : pass == 4 ? &t.material_mode
+                                         : &t.em_planes;
             if (*field == cand[k]) continue;
             *field = cand[k];
             double ms = 0;
@@ -2746,7 +3146,8 @@ void lulesh_hip_destroy(lulesh_ctx *c) {
     destroy_graphs(c);
     for (void *p : c->allocs) (void)hipFree(p);
     for (void *p : {(void *)c->d_pd, (void *)c->d_segs, (void *)c->d_zBlockProb,
-                    (void *)c->d_eBlockProb, (void *)c->d_mBlockSeg, (void *)c->d_oBlockProb})
+                    (void *)c->d_eBlockProb, (void *)c->d_mBlockSeg, (void *)c->d_oBlockProb,
+                    (void *)c->d_emBlockProb})
         if (p) (void)hipFree(p);
     if (c->hst) (void)hipHostFree(c->hst);
     for (hipEvent_t e : {c->run0, c->run1})

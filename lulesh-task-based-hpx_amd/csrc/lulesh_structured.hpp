@@ -90,6 +90,28 @@ inline void zmarch_shape(int n, int &pw, int &ph) {
     ph = take ? bh : 16;
 }
 
+// Tile of the fused elements + material pass (k_elemmat_zmarch): pw x ph
+// element columns, one thread each, with pw ph <= 256 and pw + ph <= 32 (the
+// 2 (pw + ph) elements bordering the tile are recomputed by one wave).  The
+// fewest tiles over an s x s plane, then the most threads, then the wider
+// tile.  s = 150: 17 x 15 (9 x 10 tiles instead of 10 x 10 at 16 x 16).
+inline void elemmat_shape(int s, int &pw, int &ph) {
+    int best = 0, bestThreads = 0;
+    pw = ph = 16;
+    for (int w = 4; w <= 28; ++w)
+        for (int h = 4; h <= 28; ++h) {
+            if (w * h > 256 || w + h > 32) continue;
+            const int t = ((s + w - 1) / w) * ((s + h - 1) / h);
+            if (best == 0 || t < best || (t == best && w * h > bestThreads) ||
+                (t == best && w * h == bestThreads && w > pw)) {
+                best = t;
+                bestThreads = w * h;
+                pw = w;
+                ph = h;
+            }
+        }
+}
+
 struct NodeIn {
     const double *x, *y, *z, *xd, *yd, *zd;
 };
@@ -109,7 +131,11 @@ struct ProbDesc {
     int32_t pw, ph;           // z-march patch: elements per row, rows
     int32_t tilesX, tilesY;   // z-march tiles along x and y
     int32_t zBlock0, eBlock0; // first workgroup of this problem in the z-march / element grids
-    int32_t mBlock0, pad;     // first workgroup in the element-order material grid
+    int32_t mBlock0;          // first workgroup in the element-order material grid
+    // fused elements + material pass (k_elemmat_zmarch): tile of emPW x emPH
+    // element columns, emTX x emTY tiles, emPlanes element planes per
+    // workgroup, first workgroup emBlock0
+    int32_t emPW, emPH, emTX, emTY, emPlanes, emBlock0;
     // 1/s and 1/s^2 rounded to double: element (k, j, i) from its index li
     // as floor((li + 0.5) * (1/d)), exact because the quotient stays at
     // least 0.5/s^2 (> 2^-20 for s <= 645) away from an integer while the

@@ -29,6 +29,9 @@ FIELDS = ("x y z xd yd zd xdd ydd zdd fx fy fz nodalMass e p q ql qq v volo delv
           "delx_zeta").split()
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 NODE_FIELDS = set(FIELDS[:13])
+# Q gradients: cycle temporaries, not stored by the fused elements + material
+# pass (material_mode 2)
+GRAD_FIELDS = set(FIELDS[-6:])
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int32)
@@ -62,7 +65,7 @@ class TimeState(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("zplanes", ctypes.c_int32), ("block_elem", ctypes.c_int32),
                 ("block_material", ctypes.c_int32), ("cycles_per_graph", ctypes.c_int32),
-                ("material_mode", ctypes.c_int32)]
+                ("material_mode", ctypes.c_int32), ("em_planes", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

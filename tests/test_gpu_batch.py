@@ -48,12 +48,12 @@ def test_batch_equals_individual_runs(pyl):
                                           err_msg=f"problem {k} {name}")
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_ragged_batch_with_tiny_problems(pyl, mode):
     """A batch mixing one- to eight-element problems (most regions empty)
     with larger ones, run until every problem reaches its stop time: each
     problem stops at its own cycle, and every problem's state equals the
-    same problem run alone, bit for bit, in both material kernels."""
+    same problem run alone, bit for bit, in each material mode."""
     banner(f"ragged batch, material_mode={mode}")
     params = [(1, 11, 1.0), (14, 16, 0.8), (2, 11, 1.5), (1, 1, 0.7), (9, 21, 1.2),
               (3, 21, 1.0)]
