@@ -181,14 +181,18 @@ constexpr int zmarch_waves_per_eu() {
 template <int MODE, bool STORE_F, bool SQUARE16>
 __global__ __launch_bounds__(kZmarchThreads)
 __attribute__((amdgpu_waves_per_eu(zmarch_waves_per_eu<MODE>()))) void k_nodal_zmarch(
-    const ProbDesc *__restrict__ pds, const int32_t *__restrict__ blockProb, NodeBufs nb,
-    NodeForceOut fo0, const double *__restrict__ p0,
+    const ProbDesc *__restrict__ pds, const int32_t *__restrict__ blockMap,
+    const int32_t *__restrict__ blockProb, NodeBufs nb, NodeForceOut fo0, const double *__restrict__ p0,
     const double *__restrict__ q0, const double *__restrict__ volo0,
     const double *__restrict__ v0, const double *__restrict__ ss0,
     const double *__restrict__ elemMass0, const double *__restrict__ nodalMass0, double hgcoef,
     double u_cut, DevState *states) {
     constexpr bool COMBINE = MODE != MODE_EXACT; // corner sums staged (12 values)
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    // the logical block of this workgroup (its problem, tile and plane
+    // group), or -1 for an idle padding block: the host's map orders the
+    // blocks per XCD
+    const int blk = blockMap[blockIdx.x];
+    if (blk < 0) return;
     const int prob = blockProb[blk];
     DevState *st = states + prob;
     if (!st->active) return;
