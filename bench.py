@@ -117,9 +117,15 @@ def cpu_baseline(s, nr, seconds, threads):
     return s ** 3 * cycles / el, cycles, el
 
 
-def timed_run(pyl, dev, dom, warmup, steps, tune_cycles):
+def timed_run(pyl, dev, dom, warmup, steps, tune_cycles, material_mode=None):
     """Autotune (from a snapshot that is restored), warm up, then time
-    `steps` cycles with the fields resident in HBM."""
+    `steps` cycles with the fields resident in HBM.  material_mode: the
+    material kernel to use (library default if None)."""
+    if material_mode is not None:
+        t = dev.tuning()
+        t.material_mode = material_mode
+        dev.set_tuning(t)
+        assert dev.tuning().material_mode == material_mode
     tuning = None
     if tune_cycles:
         t, _ = dev.autotune(tune_cycles)
@@ -232,6 +238,9 @@ def main():
     ap.add_argument("--batch-its", type=int, default=1000)
     ap.add_argument("--batch-streams", type=int, default=2,
                     help="streams the batch is split over (1..15)")
+    ap.add_argument("--material-mode", type=int, default=None, choices=(0, 1, 2),
+                    help="material kernel of the s=150 legs: 0 element order (the default), "
+                         "1 region lists, 2 fused with the elements phase")
     ap.add_argument("--tune-cycles", type=int, default=0,
                     help="autotune the s=150 legs first (0: the library's default launch shapes)")
     ap.add_argument("--batch-tune-cycles", type=int, default=6,
@@ -264,7 +273,8 @@ def main():
     NF = pyl.OPT_NO_NODE_FORCES
     dev = pyl.Device(dom, pyl.FP64, AO | NF)
     t_up = time.perf_counter() - t_up
-    elapsed, ts, tuning = timed_run(pyl, dev, dom, args.warmup, args.steps, args.tune_cycles)
+    elapsed, ts, tuning = timed_run(pyl, dev, dom, args.warmup, args.steps, args.tune_cycles,
+                                    args.material_mode)
     dev_ms = dev.last_run_ms()  # HIP events around the replayed graphs of the timed region
     cycles = args.steps
     zcps = E * cycles / elapsed
@@ -320,7 +330,8 @@ def main():
         oname = "exact" if args.arith == "fma" else "fma"
         domo = pyl.Domain(s, nr)
         devo = pyl.Device(domo, pyl.FP64, (pyl.OPT_FMA if oname == "fma" else 0) | NF)
-        el_o, tso, _ = timed_run(pyl, devo, domo, args.warmup, args.steps, args.tune_cycles)
+        el_o, tso, _ = timed_run(pyl, devo, domo, args.warmup, args.steps, args.tune_cycles,
+                                 args.material_mode)
         dev_ms_o = devo.last_run_ms()
         devo.set_timing(True)
         rc = devo.run(tso.cycle + PROF_CYCLES, tso)  # to the cycle the headline state is read at
@@ -348,7 +359,8 @@ def main():
     if not args.no_mixed:
         domm = pyl.Domain(s, nr)
         devm = pyl.Device(domm, pyl.MIXED, AO | NF)
-        el_m, tsm, _ = timed_run(pyl, devm, domm, args.warmup, args.steps, args.tune_cycles)
+        el_m, tsm, _ = timed_run(pyl, devm, domm, args.warmup, args.steps, args.tune_cycles,
+                                 args.material_mode)
         rc = devm.run(tsm.cycle + PROF_CYCLES, tsm)  # to the cycle the FP64 state is read at
         assert rc == 0, rc
         devm.download_view()
