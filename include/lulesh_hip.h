@@ -218,6 +218,13 @@ double lulesh_hip_last_run_ms(const lulesh_ctx *ctx);
  * two buffers.  A diagnostic with no counterpart in the reference. */
 int lulesh_hip_stream_bandwidth(int64_t bytes, int32_t reps, double *GBs);
 
+/* FP64 ceiling for the roofline: the rate (TFLOP/s, a fused multiply-add
+ * counted as two) of a hand-written kernel that issues nothing but
+ * independent v_fma_f64 (eight chains per lane, four waves per SIMD), timed
+ * over `reps` launches of about 5 ms with HIP events.  A diagnostic with no
+ * counterpart in the reference. */
+int lulesh_hip_fp64_fma_rate(int32_t reps, double *TFLOPs);
+
 /* ---------------------------------------------------------------------- */
 /* Launch tuning.  No counterpart in the reference, whose knob is the HPX
  * task size (--task-size / --elems-per-task, lulesh.cc:2267-2335). */

@@ -2918,6 +2918,66 @@ int lulesh_hip_stream_bandwidth(int64_t bytes, int32_t reps, double *GBs) {
     return rc;
 }
 
+// FP64 multiply-add issue rate: CHAINS independent fused multiply-add chains
+// per lane (x <- x a + b, bounded for 0 < a < 1), so that every SIMD always
+// has an independent v_fma_f64 to issue.  The final store never happens (no
+// sum equals the sentinel) but keeps the chains live.
+template <int CHAINS>
+__global__ __launch_bounds__(256) void k_fma_rate(double *out, int iters, double a, double b) {
+    double x[CHAINS];
+#pragma unroll
+    for (int k = 0; k < CHAINS; ++k) x[k] = double(threadIdx.x + k) * 1e-3;
+#pragma unroll 4
+    for (int i = 0; i < iters; ++i)
+#pragma unroll
+        for (int k = 0; k < CHAINS; ++k) x[k] = __builtin_fma(x[k], a, b);
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < CHAINS; ++k) s += x[k];
+    if (s == -1.0) out[blockIdx.x] = s;
+}
+
+int lulesh_hip_fp64_fma_rate(int32_t reps, double *TFLOPs) {
+    if (reps < 1 || !TFLOPs) return LULESH_INVALID_ARGUMENT;
+    *TFLOPs = 0;
+    constexpr int kChains = 8, kIters = 16384;
+    double *out = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    auto fail = [&](hipError_t e) {
+        if (rc == 0 && e != hipSuccess) rc = LULESH_HIP_ERROR - int(e);
+        return rc != 0;
+    };
+    int dev = 0, cus = 256;
+    if (!fail(hipGetDevice(&dev)) &&
+        !fail(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev))) {
+        const int blocks = 16 * cus; // four waves per SIMD, four rounds
+        if (!fail(hipMalloc(&out, size_t(blocks) * sizeof(double))) &&
+            !fail(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) &&
+            !fail(hipEventCreate(&e0)) && !fail(hipEventCreate(&e1))) {
+            const double a = 0.999999, b = 1e-6;
+            k_fma_rate<kChains><<<blocks, 256, 0, st>>>(out, kIters, a, b); // warm-up
+            if (!fail(hipEventRecord(e0, st))) {
+                for (int r = 0; r < reps; ++r)
+                    k_fma_rate<kChains><<<blocks, 256, 0, st>>>(out, kIters, a, b);
+                float ms = 0;
+                if (!fail(hipGetLastError()) && !fail(hipEventRecord(e1, st)) &&
+                    !fail(hipEventSynchronize(e1)) && !fail(hipEventElapsedTime(&ms, e0, e1)) &&
+                    ms > 0)
+                    *TFLOPs = 2.0 * kChains * double(kIters) * 256.0 * blocks * reps /
+                              (ms * 1e-3) / 1e12;
+            }
+        }
+    }
+    if (st) (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (out) (void)hipFree(out);
+    return rc;
+}
+
 int lulesh_hip_download(lulesh_ctx *c, int id, void *dst, size_t count) {
     return field_copy(c, 0, id, dst, count, true);
 }

@@ -151,6 +151,9 @@ double lulesh_hip_last_run_ms(const lulesh_ctx *c) {
 int lulesh_hip_stream_bandwidth(int64_t bytes, int32_t reps, double *GBs) {
     return ex::lulesh_hip_stream_bandwidth(bytes, reps, GBs);
 }
+int lulesh_hip_fp64_fma_rate(int32_t reps, double *TFLOPs) {
+    return ex::lulesh_hip_fp64_fma_rate(reps, TFLOPs);
+}
 int lulesh_hip_constraints_noalloc(const double *ss, const double *vdov, const double *arealg,
                                    int32_t numElem, double qqc, double dvovmax, double *out,
                                    void *stream) {

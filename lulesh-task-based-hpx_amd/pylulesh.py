@@ -133,6 +133,7 @@ def lib():
                                                          ctypes.POINTER(vp)]),
         "lulesh_hip_report": (ctypes.c_int, [vp, _D]),
         "lulesh_hip_stream_bandwidth": (ctypes.c_int, [ctypes.c_int64, i32, _D]),
+        "lulesh_hip_fp64_fma_rate": (ctypes.c_int, [i32, _D]),
         "lulesh_domain_write_viz": (ctypes.c_int, [vp, ctypes.c_char_p, i32]),
     }
     for name, (res, args) in more.items():
