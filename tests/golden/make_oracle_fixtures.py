@@ -4,6 +4,7 @@ not depend on the thread count) run over the benchmark configurations at
 their full iteration counts.
 
   s150_r11_i2000   s=150, 11 regions, -b 1 -c 1, 2000 cycles
+  s150_r11_stop    the same problem run to its stop time (t = 1e-2, 5382 cycles)
   s250_r21_i500    s=250, 21 regions, -b 1 -c 1, 500 cycles
   batch64_p<k>     problem k of the seeded 64-problem ensemble (seed 0:
                    sizes in [20, 80], regions in {11, 16, 21}, deposit
@@ -17,7 +18,8 @@ digits, and per state field: max |f|, sum f, sum |f|, sum f^2 and the values
 at a fixed set of indices (every element / node of the cube next to the
 origin, where the blast is, and a seeded random sample of the rest).
 
-Usage (in the build container, a few tens of minutes on 8 cores):
+Usage (in the build container, a few tens of minutes on 8 cores; s150_r11_stop
+about 15 minutes alone):
     python tests/golden/make_oracle_fixtures.py [name ...]
 """
 import json
@@ -42,6 +44,7 @@ BATCH_MEMBERS = (40, 17, 26)
 def configs():
     out = {
         "s150_r11_i2000": dict(s=150, nr=11, balance=1, cost=1, its=2000, factor=1.0),
+        "s150_r11_stop": dict(s=150, nr=11, balance=1, cost=1, its=10 ** 7, factor=1.0),
         "s250_r21_i500": dict(s=250, nr=21, balance=1, cost=1, its=500, factor=1.0),
     }
     sys.path.insert(0, os.path.join(ROOT, "lulesh-task-based-hpx_amd"))
