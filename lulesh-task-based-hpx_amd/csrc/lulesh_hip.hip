@@ -1592,11 +1592,12 @@ int enqueue_nodal(lulesh_ctx *c) {
                               c->f[LULESH_F_XDD], c->f[LULESH_F_YDD], c->f[LULESH_F_ZDD]};
         return launch(c, T_NODAL, [&] {
             constexpr int M = sizeof(R) == 4 ? MODE_MIXED : kFusedForm ? MODE_FUSED : MODE_EXACT;
+            constexpr bool D = M == MODE_FUSED; // both corner sets staged at once
             auto kern = c->zSquare16
-                            ? (c->store_forces ? k_nodal_zmarch<M, true, true>
-                                               : k_nodal_zmarch<M, false, true>)
-                            : (c->store_forces ? k_nodal_zmarch<M, true, false>
-                                               : k_nodal_zmarch<M, false, false>);
+                            ? (c->store_forces ? k_nodal_zmarch<M, true, true, D>
+                                               : k_nodal_zmarch<M, false, true, D>)
+                            : (c->store_forces ? k_nodal_zmarch<M, true, false, D>
+                                               : k_nodal_zmarch<M, false, false, D>);
             kern<<<c->zGrid, kZmarchThreads, 0, c->stream>>>(
                 c->d_pd, c->d_zMap, c->d_zBlockProb, c->nb, fo, c->f[LULESH_F_P], c->f[LULESH_F_Q],
                 c->f[LULESH_F_VOLO], c->f[LULESH_F_V], c->f[LULESH_F_SS], c->f[LULESH_F_ELEMMASS],

@@ -178,7 +178,7 @@ constexpr int zmarch_waves_per_eu() {
     return MODE == MODE_MIXED ? 3 : 2;
 }
 
-template <int MODE, bool STORE_F, bool SQUARE16>
+template <int MODE, bool STORE_F, bool SQUARE16, bool DBL>
 __global__ __launch_bounds__(kZmarchThreads)
 __attribute__((amdgpu_waves_per_eu(zmarch_waves_per_eu<MODE>()))) void k_nodal_zmarch(
     const ProbDesc *__restrict__ pds, const int32_t *__restrict__ blockMap,
@@ -218,7 +218,10 @@ __attribute__((amdgpu_waves_per_eu(zmarch_waves_per_eu<MODE>()))) void k_nodal_z
     // LDS: the staged corner values of the element plane (4 corners x 3
     // stress + 3 hourglass values, or 3 sums), and the node-plane ring.
     constexpr int kStage = COMBINE ? 12 : 24; // staged values per element
-    __shared__ double lds[kStage][kZmarchThreads];
+    // DBL: the upper corners get their own staging rows (lds[kStage..]), so
+    // both corner sets are staged before one barrier and read after it
+    static_assert(!DBL || COMBINE, "two staging areas only for the combined corner sums");
+    __shared__ double lds[DBL ? 2 * kStage : kStage][kZmarchThreads];
     __shared__ double ring[2][6 * kPatchNodes];
     auto slot_of = [](int K) { return K & 1; };
     auto pnv = [&](int slot, int f, int t) -> double & { return ring[slot][f * kPatchNodes + t]; };
@@ -250,9 +253,16 @@ __attribute__((amdgpu_waves_per_eu(zmarch_waves_per_eu<MODE>()))) void k_nodal_z
     double px = 0.0, py = 0.0, pz = 0.0; // partial sums of the node in the current plane
 
     const int own = (jj + dy) * (pw + 1) + (ii + dx); // node (J, I) in the patch node array
-    auto finish_node = [&](int K, double mass) {
+    // the node's start-of-cycle state (x, y, z, xd, yd, zd) from the ring;
+    // DBL reads it during the element work, before the barrier after which
+    // the ring slot may be refilled, the other forms when they use it
+    auto own_state = [&](int K, double ox[6]) {
+#pragma unroll
+        for (int f = 0; f < 6; ++f) ox[f] = pnv(slot_of(K), f, own);
+    };
+    // old(f): field f of the node's start-of-cycle state
+    auto finish_node = [&](int K, double mass, auto old) {
         const int g = (K * n + J) * n + I;
-        const int slot = slot_of(K);
         double ax, ay, az;
         if constexpr (MODE == MODE_FUSED) {
             // one division and three products (not bitwise the quotients)
@@ -278,18 +288,18 @@ __attribute__((amdgpu_waves_per_eu(zmarch_waves_per_eu<MODE>()))) void k_nodal_z
             fo.ydd[g] = ay;
             fo.zdd[g] = az;
         }
-        double vx = pnv(slot, 3, own) + ax * dt;
+        double vx = old(3) + ax * dt;
         if (fabs_(vx) < u_cut) vx = 0.0;
-        double vy = pnv(slot, 4, own) + ay * dt;
+        double vy = old(4) + ay * dt;
         if (fabs_(vy) < u_cut) vy = 0.0;
-        double vz = pnv(slot, 5, own) + az * dt;
+        double vz = old(5) + az * dt;
         if (fabs_(vz) < u_cut) vz = 0.0;
         out.xd[g] = vx;
         out.yd[g] = vy;
         out.zd[g] = vz;
-        out.x[g] = pnv(slot, 0, own) + vx * dt;
-        out.y[g] = pnv(slot, 1, own) + vy * dt;
-        out.z[g] = pnv(slot, 2, own) + vz * dt;
+        out.x[g] = old(0) + vx * dt;
+        out.y[g] = old(1) + vy * dt;
+        out.z[g] = old(2) + vz * dt;
     };
     // Ring fill: node plane K goes into its slot in two steps: fetch_plane
     // issues the global loads into registers (coalesced rows of pw+1 nodes;
@@ -354,11 +364,11 @@ __attribute__((amdgpu_waves_per_eu(zmarch_waves_per_eu<MODE>()))) void k_nodal_z
     };
     // add the corner staged at slot c of patch thread t: stress then
     // hourglass, or their sum (COMBINE)
-    auto add = [&](int c, int t) {
+    auto add = [&](int c, int t, int base = 0) { // base: kStage for the upper rows (DBL)
         if constexpr (COMBINE) {
-            px += lds[3 * c + 0][t];
-            py += lds[3 * c + 1][t];
-            pz += lds[3 * c + 2][t];
+            px += lds[base + 3 * c + 0][t];
+            py += lds[base + 3 * c + 1][t];
+            pz += lds[base + 3 * c + 2][t];
         } else {
             px += lds[6 * c + 0][t];
             px += lds[6 * c + 3][t];
@@ -383,12 +393,17 @@ __attribute__((amdgpu_waves_per_eu(zmarch_waves_per_eu<MODE>()))) void k_nodal_z
     double mcur = fetch_mass(kbeg);
     for (int k = kbeg; k < kend; ++k) {
         // Plane k is in its slot; plane k+1 (fetched during the previous
-        // step) goes over plane k-1, whose last readers (finish_node) passed
-        // the previous step's third barrier.  The previous step's last phase
-        // only read the corner staging area, which is written again after
-        // this barrier.
+        // step) goes over plane k-1, whose last readers passed the previous
+        // step's second barrier (DBL: the previous element work and own_state
+        // read it; else: finish_node, before the third).  The previous step's
+        // last phase only read the corner staging area, which is written
+        // again after this barrier.
         store_plane(k + 1);
         __syncthreads();
+        double ox[6]; // (DBL) this thread's node of plane k, read while its slot is stable
+        if constexpr (DBL) {
+            if (ownsNode && k >= K0) own_state(k, ox);
+        }
         // Wave priorities (s_setprio) for the workgroups sharing each SIMD:
         // the element work issues ahead of the other workgroup's node phases
         // (its LDS-bound corner sums and node updates fill the issue slots
@@ -507,6 +522,12 @@ __attribute__((amdgpu_waves_per_eu(zmarch_waves_per_eu<MODE>()))) void k_nodal_z
 #pragma unroll
             for (int c = 0; c < kStage; ++c) up[c] = 0.0;
         }
+        if constexpr (DBL) { // the upper corners to their own rows now
+            if (elemCol) {
+#pragma unroll
+                for (int c = 0; c < kStage; ++c) lds[kStage + c][tid] = up[c];
+            }
+        }
         __builtin_amdgcn_sched_barrier(0); // keep the prefetch out of the element work
         // the next step's plane, element scalars and nodal mass
         fetch_plane(k + 2, k + 1 < kend);
@@ -520,27 +541,35 @@ __attribute__((amdgpu_waves_per_eu(zmarch_waves_per_eu<MODE>()))) void k_nodal_z
             if (v01) add(3, tb + 1);
             if (v10) add(1, tb + pw);
             if (v11) add(0, tb + pw + 1);
-            finish_node(k, mcur);
+            if constexpr (DBL)
+                finish_node(k, mcur, [&](int f) { return ox[f]; });
+            else
+                finish_node(k, mcur, [&](int f) { return pnv(slot_of(k), f, own); });
         }
-        __syncthreads();
-        if (elemCol) {
+        if constexpr (!DBL) { // the upper corners over the lower ones, between barriers
+            __syncthreads();
+            if (elemCol) {
 #pragma unroll
-            for (int c = 0; c < kStage; ++c) lds[c][tid] = up[c];
+                for (int c = 0; c < kStage; ++c) lds[c][tid] = up[c];
+            }
+            __syncthreads();
         }
-        __syncthreads();
         // ---- node plane k+1 starts with corners 6,7,5,4 -------------------------
-        // lds now holds corners 4..7 at slots 0..3
+        // the upper corners' rows (lds[kStage..] with DBL, else lds[0..])
+        // hold corners 4..7 at slots 0..3
+        constexpr int kUp = DBL ? kStage : 0;
         px = py = pz = 0.0;
         if (ownsNode && k + 1 < K1) {
-            if (v00) add(2, tb);          // corner 6 of element (J-1, I-1)
-            if (v01) add(3, tb + 1);      // corner 7 of element (J-1, I)
-            if (v10) add(1, tb + pw);     // corner 5 of element (J, I-1)
-            if (v11) add(0, tb + pw + 1); // corner 4 of element (J, I)
+            if (v00) add(2, tb, kUp);          // corner 6 of element (J-1, I-1)
+            if (v01) add(3, tb + 1, kUp);      // corner 7 of element (J-1, I)
+            if (v10) add(1, tb + pw, kUp);     // corner 5 of element (J, I-1)
+            if (v11) add(0, tb + pw + 1, kUp); // corner 4 of element (J, I)
         }
         mcur = mnext;
     }
     // The top node plane (K = s) has no element plane above it.
-    if (ownsNode && K1 == n && kend == s && s >= K0) finish_node(s, mcur);
+    if (ownsNode && K1 == n && kend == s && s >= K0)
+        finish_node(s, mcur, [&](int f) { return pnv(slot_of(s), f, own); });
 }
 
 } // namespace lulesh_dev
