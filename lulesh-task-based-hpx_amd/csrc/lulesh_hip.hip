@@ -1592,7 +1592,7 @@ int enqueue_nodal(lulesh_ctx *c) {
                               c->f[LULESH_F_XDD], c->f[LULESH_F_YDD], c->f[LULESH_F_ZDD]};
         return launch(c, T_NODAL, [&] {
             constexpr int M = sizeof(R) == 4 ? MODE_MIXED : kFusedForm ? MODE_FUSED : MODE_EXACT;
-            constexpr bool D = M == MODE_FUSED; // both corner sets staged at once
+            constexpr bool D = M != MODE_EXACT; // both corner sets staged at once
             auto kern = c->zSquare16
                             ? (c->store_forces ? k_nodal_zmarch<M, true, true, D>
                                                : k_nodal_zmarch<M, false, true, D>)

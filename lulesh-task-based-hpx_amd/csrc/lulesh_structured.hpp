@@ -156,13 +156,18 @@ struct ProbDesc {
 //    VGPRs, held to 256 (a handful of spilled values): two workgroups per CU.
 //  MODE_FUSED (FP64, multiply-adds fused): the element adds its stress and
 //    hourglass force per corner before staging (one more rounding per
-//    corner, no change in what is summed), 12 staged values per element
-//    (52 KB), and forms the hourglass force in two register-lean passes:
-//    216 VGPRs without spills, half the LDS traffic of the node phases.  Two
-//    workgroups per CU.  At s=150 the nodal kernel takes 0.185 ms against
-//    0.211 ms staged as MODE_EXACT.
-//  MODE_MIXED (FP32 hourglass): staged as MODE_FUSED, 148 VGPRs and 52 KB:
-//    three workgroups per CU (0.178 ms against 0.212 ms staged separately).
+//    corner, no change in what is summed), 12 staged values per element,
+//    and forms the hourglass force in two register-lean passes: 228 VGPRs
+//    without spills.  Two workgroups per CU.  At s=150 the nodal kernel takes
+//    0.185 ms against 0.211 ms staged as MODE_EXACT.
+//  MODE_MIXED (FP32 hourglass): staged as MODE_FUSED, 166 VGPRs.
+//  DBL (the fused and mixed forms): the upper corners get staging rows of
+//    their own (77 KB of LDS with the ring), so both corner sets are staged
+//    before one barrier, and each thread reads its node's old state from the
+//    ring during the element work: two barriers per plane instead of four.
+//    Two workgroups per CU (the VGPRs of the fused form allow no more; the
+//    mixed form had three at 52 KB).  s=150: fused 0.183 -> 0.180 ms, mixed
+//    0.177 -> 0.169 ms.
 // Tried and dropped: FP64 staged combined at three workgroups per CU (needs
 // more than the 168 VGPRs of three waves per SIMD and spills: 0.48-0.59
 // ms); the next plane copied into LDS by global_load_lds during the element
@@ -173,14 +178,10 @@ struct ProbDesc {
 // then a compile-time constant; runtime shapes cost 2-6% at s=150).
 enum ZmarchMode { MODE_EXACT = 0, MODE_FUSED = 1, MODE_MIXED = 2 };
 
-template <int MODE>
-constexpr int zmarch_waves_per_eu() {
-    return MODE == MODE_MIXED ? 3 : 2;
-}
 
 template <int MODE, bool STORE_F, bool SQUARE16, bool DBL>
 __global__ __launch_bounds__(kZmarchThreads)
-__attribute__((amdgpu_waves_per_eu(zmarch_waves_per_eu<MODE>()))) void k_nodal_zmarch(
+__attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
     const ProbDesc *__restrict__ pds, const int32_t *__restrict__ blockMap,
     const int32_t *__restrict__ blockProb, NodeBufs nb, NodeForceOut fo0, const double *__restrict__ p0,
     const double *__restrict__ q0, const double *__restrict__ volo0,
