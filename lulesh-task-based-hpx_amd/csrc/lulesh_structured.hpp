@@ -222,6 +222,10 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
     // DBL: the upper corners get their own staging rows (lds[kStage..]), so
     // both corner sets are staged before one barrier and read after it
     static_assert(!DBL || COMBINE, "two staging areas only for the combined corner sums");
+    // LOREG: the lower corners' sums are kept in registers too and staged with
+    // the upper ones (the mixed form: 160 against 168 us at s=150; the fused
+    // form's VGPRs would reach 252 and it ran 3% slower)
+    constexpr bool LOREG = DBL && MODE == MODE_MIXED;
     __shared__ double lds[DBL ? 2 * kStage : kStage][kZmarchThreads];
     __shared__ double ring[2][6 * kPatchNodes];
     auto slot_of = [](int K) { return K & 1; };
@@ -414,6 +418,14 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
         __builtin_amdgcn_s_setprio(2);
         // ---- element (k, ej, ei): stress + hourglass corner forces ----------
         double up[kStage]; // corners 4..7: (sx, sy, sz, hx, hy, hz) each, or their sums
+        double lo[kStage]; // (LOREG) corners 0..3 likewise, staged with the upper ones
+        // the running sum of lower-corner value c: in registers (LOREG) or in LDS
+        auto lower = [&](int c) -> double & {
+            if constexpr (LOREG)
+                return lo[c];
+            else
+                return lds[c][tid];
+        };
         if (elemCol) {
             // sigxx = sigyy = sigzz = -p - q (lulesh.cc:629-633) and the
             // hourglass determinant volo*v (lulesh.cc:1098)
@@ -447,7 +459,10 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
                 for (int a = 0; a < 4; ++a)
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
-                        lds[W * a + d][tid] = -(sig * sc[d][a]);
+                        if constexpr (LOREG)
+                            lo[W * a + d] = -(sig * sc[d][a]);
+                        else
+                            lds[W * a + d][tid] = -(sig * sc[d][a]);
                         up[W * a + d] = -(sig * sc[d][a + 4]);
                     }
             }
@@ -478,7 +493,7 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
                     [&](int c, int a) { return float(node(3 + c, a)); },
                     [&](int c, int a, float f) {
                         if (a < 4)
-                            lds[3 * a + c][tid] += double(f);
+                            lower(3 * a + c) += double(f);
                         else
                             up[3 * (a - 4) + c] += double(f);
                     });
@@ -495,7 +510,7 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
                     [&](int c, int a) { return node(3 + c, a); },
                     [&](int c, int a, double f) {
                         if (a < 4)
-                            lds[3 * a + c][tid] += f;
+                            lower(3 * a + c) += f;
                         else
                             up[3 * (a - 4) + c] += f;
                     });
@@ -523,10 +538,13 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
 #pragma unroll
             for (int c = 0; c < kStage; ++c) up[c] = 0.0;
         }
-        if constexpr (DBL) { // the upper corners to their own rows now
+        if constexpr (DBL) { // the upper corners (LOREG: both sets) to their rows now
             if (elemCol) {
 #pragma unroll
-                for (int c = 0; c < kStage; ++c) lds[kStage + c][tid] = up[c];
+                for (int c = 0; c < kStage; ++c) {
+                    if constexpr (LOREG) lds[c][tid] = lo[c];
+                    lds[kStage + c][tid] = up[c];
+                }
             }
         }
         __builtin_amdgcn_sched_barrier(0); // keep the prefetch out of the element work
