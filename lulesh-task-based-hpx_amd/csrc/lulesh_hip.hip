@@ -248,6 +248,21 @@ __device__ __forceinline__ void load8(const double *__restrict__ f, const int n[
     for (int k = 0; k < 8; ++k) o[k] = f[n[k]];
 }
 
+// The eight node values of a structured element in four 16-byte loads: its
+// nodes pair up along x (n1 = n0 + 1, n2 = n3 + 1, n5 = n4 + 1, n6 = n7 + 1).
+// A pair is 8-byte aligned; gfx950 reads it with one dwordx4 load.
+typedef double node_pair __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void load8_pairs(const double *__restrict__ f, const int n[8],
+                                            double o[8]) {
+    node_pair p[4];
+    __builtin_memcpy(&p[0], f + n[0], sizeof(node_pair));
+    __builtin_memcpy(&p[1], f + n[3], sizeof(node_pair));
+    __builtin_memcpy(&p[2], f + n[4], sizeof(node_pair));
+    __builtin_memcpy(&p[3], f + n[7], sizeof(node_pair));
+    o[0] = p[0].x; o[1] = p[0].y; o[3] = p[1].x; o[2] = p[1].y;
+    o[4] = p[2].x; o[5] = p[2].y; o[7] = p[3].x; o[6] = p[3].y;
+}
+
 __device__ __forceinline__ void load_nodelist(const int32_t *__restrict__ nodelist, int i,
                                               int n[8]) {
     const int4 a = reinterpret_cast<const int4 *>(nodelist)[2 * i];
@@ -453,15 +468,21 @@ __global__ __launch_bounds__(kBlock) void k_elems(
     } else {
         load_nodelist(nodelist, li, n);
     }
+    auto load8_node = [&](const double *f, const int nd[8], double o[8]) {
+        if constexpr (STRUCT)
+            load8_pairs(f, nd, o);
+        else
+            load8(f, nd, o);
+    };
     double x[8], y[8], z[8], xd[8], yd[8], zd[8];
     // every load is issued before the first store (the stores could alias
     // the node arrays as far as the compiler knows)
-    load8(nf[0], n, x);
-    load8(nf[1], n, y);
-    load8(nf[2], n, z);
-    load8(nf[3], n, xd);
-    load8(nf[4], n, yd);
-    load8(nf[5], n, zd);
+    load8_node(nf[0], n, x);
+    load8_node(nf[1], n, y);
+    load8_node(nf[2], n, z);
+    load8_node(nf[3], n, xd);
+    load8_node(nf[4], n, yd);
+    load8_node(nf[5], n, zd);
     const double voloi = f0.volo[i];
     const double vold = f0.v[i];
     const double volume = elem_volume<double>(x, y, z);
@@ -903,12 +924,12 @@ __global__ __launch_bounds__(kEmThreads) void k_elemmat_zmarch(
         int n[8];
         nodes_of(k, j, i, n);
         double x[8], y[8], z[8], xd[8], yd[8], zd[8];
-        load8(nf[0], n, x);
-        load8(nf[1], n, y);
-        load8(nf[2], n, z);
-        load8(nf[3], n, xd);
-        load8(nf[4], n, yd);
-        load8(nf[5], n, zd);
+        load8_pairs(nf[0], n, x);
+        load8_pairs(nf[1], n, y);
+        load8_pairs(nf[2], n, z);
+        load8_pairs(nf[3], n, xd);
+        load8_pairs(nf[4], n, yd);
+        load8_pairs(nf[5], n, zd);
         const double voloi = f0.volo[eo + (k * s + j) * s + i];
         const double rel = elem_volume<double>(x, y, z) / voloi;
         element_q_gradients<QG>(x, y, z, xd, yd, zd, voloi * rel, g);
@@ -931,12 +952,12 @@ __global__ __launch_bounds__(kEmThreads) void k_elemmat_zmarch(
                 int n[8];
                 nodes_of(t, ej, ei, n);
                 double x[8], y[8], z[8], xd[8], yd[8], zd[8];
-                load8(nf[0], n, x);
-                load8(nf[1], n, y);
-                load8(nf[2], n, z);
-                load8(nf[3], n, xd);
-                load8(nf[4], n, yd);
-                load8(nf[5], n, zd);
+                load8_pairs(nf[0], n, x);
+                load8_pairs(nf[1], n, y);
+                load8_pairs(nf[2], n, z);
+                load8_pairs(nf[3], n, xd);
+                load8_pairs(nf[4], n, yd);
+                load8_pairs(nf[5], n, zd);
                 const double voloi = f0.volo[i];
                 const double vold = f0.v[i];
                 const double volume = elem_volume<double>(x, y, z);
