@@ -231,7 +231,8 @@ int lulesh_hip_fp64_fma_rate(int32_t reps, double *TFLOPs);
 typedef struct lulesh_tuning {
     int32_t zplanes;          /* node planes per z-march workgroup (0 = sized to fill the GPU) */
     int32_t block_elem;       /* threads per block of the kinematics kernel: 64, 128, 256
-                                 (other values: 128) */
+                                 (other values: 256 for a context of one problem, 128
+                                 for a batch) */
     int32_t block_material;   /* threads per block of the material/EOS kernel: 64, 128, 256
                                  (other values: 64 for a context of one problem, 128
                                  for a batch) */

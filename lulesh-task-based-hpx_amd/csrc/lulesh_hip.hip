@@ -1510,7 +1510,10 @@ std::vector<int32_t> zmarch_block_map(const std::vector<ProbDesc> &pds, int nblo
 // Rebuild the block -> problem / segment tables for the current tuning.
 int configure(lulesh_ctx *c) {
     lulesh_tuning &t = c->tune;
-    if (t.block_elem != 64 && t.block_elem != 128 && t.block_elem != 256) t.block_elem = 128;
+    // kinematics workgroups: 256 threads for one problem (the autotuner's pick
+    // at s = 150: 137 against 139.5 us for 128), 128 for a batch
+    if (t.block_elem != 64 && t.block_elem != 128 && t.block_elem != 256)
+        t.block_elem = c->nprob == 1 ? 256 : 128;
     // material workgroups: 64 threads for one problem (s = 150: 0.3-1.1% less
     // time per cycle than 128 in each arithmetic form), 128 for a batch
     if (t.block_material != 64 && t.block_material != 128 && t.block_material != 256)

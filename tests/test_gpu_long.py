@@ -4,7 +4,8 @@ The oracle fixtures (tests/golden/oracle_*.json + .npz, written by
 tests/golden/make_oracle_fixtures.py from the CPU oracle) hold the final
 state of
 
-  * s=150, 11 regions, -b 1 -c 1, 2000 cycles (the headline problem), and
+  * s=150, 11 regions, -b 1 -c 1, 2000 cycles (the headline problem),
+  * the same problem run to its stop time (t = 1e-2, 5382 cycles), and
   * s=250, 21 regions, -b 1 -c 1, 500 cycles (region imbalance at scale):
 
 the time state, the report numbers with 17 digits, per state field its
@@ -76,7 +77,7 @@ FORMS = {"exact": (0, 0), "fused": (0, 0x400), "mixed": (1, 0x400)}
 
 
 @pytest.mark.parametrize("form", ["exact", "fused", "mixed"])
-@pytest.mark.parametrize("name", ["s150_r11_i2000", "s250_r21_i500"])
+@pytest.mark.parametrize("name", ["s150_r11_i2000", "s150_r11_stop", "s250_r21_i500"])
 def test_full_length_run_vs_oracle(pyl, name, form):
     meta, arrays = load_fixture(name)
     prec, opts = FORMS[form]
