@@ -239,6 +239,61 @@ __device__ __forceinline__ void node_normals(double pf[3][8], const double x[8],
     face_normal(pf, 4, 7, 6, 5, x, y, z);
 }
 
+// The stress integration (lulesh.cc:805-830) with the power-of-two factors
+// taken out.  Multiplying by a power of two is exact and commutes with every
+// rounding (no value comes near the under- or overflow range), so these give
+// the reference's results, bit for bit, without those multiplies:
+//  * jacobian_det_x64: 64 times the determinant shape_derivatives returns
+//    (the .125 of each Jacobian column term and the final 8 dropped; the
+//    stress path only tests it against zero, lulesh.cc:819-822);
+//  * node_normals_x16: 16 times node_normals (each bisector's 0.5 and each
+//    face area's 0.25 dropped), so that sig * pf = (sig / 16) * pf16 exactly.
+__device__ __forceinline__ double jacobian_det_x64(const double x[8], const double y[8],
+                                                   const double z[8]) {
+    const double fjxxi = (x[6] - x[0]) + (x[5] - x[3]) - (x[7] - x[1]) - (x[4] - x[2]);
+    const double fjxet = (x[6] - x[0]) - (x[5] - x[3]) + (x[7] - x[1]) - (x[4] - x[2]);
+    const double fjxze = (x[6] - x[0]) + (x[5] - x[3]) + (x[7] - x[1]) + (x[4] - x[2]);
+    const double fjyxi = (y[6] - y[0]) + (y[5] - y[3]) - (y[7] - y[1]) - (y[4] - y[2]);
+    const double fjyet = (y[6] - y[0]) - (y[5] - y[3]) + (y[7] - y[1]) - (y[4] - y[2]);
+    const double fjyze = (y[6] - y[0]) + (y[5] - y[3]) + (y[7] - y[1]) + (y[4] - y[2]);
+    const double fjzxi = (z[6] - z[0]) + (z[5] - z[3]) - (z[7] - z[1]) - (z[4] - z[2]);
+    const double fjzet = (z[6] - z[0]) - (z[5] - z[3]) + (z[7] - z[1]) - (z[4] - z[2]);
+    const double fjzze = (z[6] - z[0]) + (z[5] - z[3]) + (z[7] - z[1]) + (z[4] - z[2]);
+    const double cjxet = -(fjyxi * fjzze) + (fjzxi * fjyze);
+    const double cjyet = (fjxxi * fjzze) - (fjzxi * fjxze);
+    const double cjzet = -(fjxxi * fjyze) + (fjyxi * fjxze);
+    return fjxet * cjxet + fjyet * cjyet + fjzet * cjzet;
+}
+
+__device__ __forceinline__ void face_normal_x16(double pf[3][8], int n0, int n1, int n2, int n3,
+                                                const double x[8], const double y[8],
+                                                const double z[8]) {
+    const double bx0 = x[n3] + x[n2] - x[n1] - x[n0];
+    const double by0 = y[n3] + y[n2] - y[n1] - y[n0];
+    const double bz0 = z[n3] + z[n2] - z[n1] - z[n0];
+    const double bx1 = x[n2] + x[n1] - x[n3] - x[n0];
+    const double by1 = y[n2] + y[n1] - y[n3] - y[n0];
+    const double bz1 = z[n2] + z[n1] - z[n3] - z[n0];
+    const double ax = by0 * bz1 - bz0 * by1;
+    const double ay = bz0 * bx1 - bx0 * bz1;
+    const double az = bx0 * by1 - by0 * bx1;
+    pf[0][n0] += ax; pf[0][n1] += ax; pf[0][n2] += ax; pf[0][n3] += ax;
+    pf[1][n0] += ay; pf[1][n1] += ay; pf[1][n2] += ay; pf[1][n3] += ay;
+    pf[2][n0] += az; pf[2][n1] += az; pf[2][n2] += az; pf[2][n3] += az;
+}
+
+__device__ __forceinline__ void node_normals_x16(double pf[3][8], const double x[8],
+                                                 const double y[8], const double z[8]) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) pf[0][i] = pf[1][i] = pf[2][i] = 0.0;
+    face_normal_x16(pf, 0, 1, 2, 3, x, y, z);
+    face_normal_x16(pf, 0, 4, 5, 1, x, y, z);
+    face_normal_x16(pf, 1, 5, 6, 2, x, y, z);
+    face_normal_x16(pf, 2, 6, 7, 3, x, y, z);
+    face_normal_x16(pf, 3, 7, 4, 0, x, y, z);
+    face_normal_x16(pf, 4, 7, 6, 5, x, y, z);
+}
+
 // VoluDer, lulesh.cc:468-491
 template <typename T>
 __device__ __forceinline__ void volu_der(T x0, T x1, T x2, T x3, T x4, T x5, T y0, T y1, T y2,

@@ -297,17 +297,17 @@ __global__ __launch_bounds__(kBlock) void k_force(
     load8(nf.y, n, y);
     load8(nf.z, n, z);
     {
-        const double det = shape_derivatives<false>(x, y, z, nullptr);
-        if (det == 0) st->error = kVolumeError; // lulesh.cc:819-822
+        // (the scaled forms give the reference's values bit for bit)
+        if (jacobian_det_x64(x, y, z) == 0) st->error = kVolumeError; // lulesh.cc:819-822
         double B[3][8];
-        node_normals(B, x, y, z);
-        const double sig = -p[i] - q[i];
+        node_normals_x16(B, x, y, z);
+        const double sig16 = (-p[i] - q[i]) * 0.0625;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const size_t o = size_t(k) * E + i;
-            csx[o] = -(sig * B[0][k]);
-            csy[o] = -(sig * B[1][k]);
-            csz[o] = -(sig * B[2][k]);
+            csx[o] = -(sig16 * B[0][k]);
+            csy[o] = -(sig16 * B[1][k]);
+            csz[o] = -(sig16 * B[2][k]);
         }
     }
     const double determ = volo[i] * v[i];

@@ -112,6 +112,34 @@ inline void elemmat_shape(int s, int &pw, int &ph) {
         }
 }
 
+// Element i of an array whose base is the same for the whole workgroup, as
+// a 32-bit byte offset from that base: the access then keeps the base in
+// scalar registers and only the offset in a vector register (the saddr form
+// of global_load / global_store) instead of a 64-bit address per access.
+// The arrays of one problem stay below 4 GiB (at most 646^3 values of 8
+// bytes).
+#ifndef LULESH_SADDR
+#define LULESH_SADDR 1
+#endif
+template <typename T>
+__device__ __forceinline__ T &at32(T *base, int i) {
+#if LULESH_SADDR
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) +
+                                  uint32_t(i) * uint32_t(sizeof(T)));
+#else
+    return base[i];
+#endif
+}
+template <typename T>
+__device__ __forceinline__ const T &at32(const T *base, int i) {
+#if LULESH_SADDR
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) +
+                                        uint32_t(i) * uint32_t(sizeof(T)));
+#else
+    return base[i];
+#endif
+}
+
 struct NodeIn {
     const double *x, *y, *z, *xd, *yd, *zd;
 };
@@ -177,7 +205,6 @@ struct ProbDesc {
 // SQUARE16: every problem of the launch has 16 x 16 patches (the shape is
 // then a compile-time constant; runtime shapes cost 2-6% at s=150).
 enum ZmarchMode { MODE_EXACT = 0, MODE_FUSED = 1, MODE_MIXED = 2 };
-
 
 template <int MODE, bool STORE_F, bool SQUARE16, bool DBL>
 __global__ __launch_bounds__(kZmarchThreads)
@@ -281,17 +308,17 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
             az = pz / mass;
         }
         if (STORE_F) {
-            fo.fx[g] = px;
-            fo.fy[g] = py;
-            fo.fz[g] = pz;
+            at32(fo.fx, g) = px;
+            at32(fo.fy, g) = py;
+            at32(fo.fz, g) = pz;
         }
         if (I == 0) ax = 0.0; // symmX: the x = 0 plane
         if (J == 0) ay = 0.0; // symmY
         if (K == 0) az = 0.0; // symmZ
         if (STORE_F) {
-            fo.xdd[g] = ax;
-            fo.ydd[g] = ay;
-            fo.zdd[g] = az;
+            at32(fo.xdd, g) = ax;
+            at32(fo.ydd, g) = ay;
+            at32(fo.zdd, g) = az;
         }
         double vx = old(3) + ax * dt;
         if (fabs_(vx) < u_cut) vx = 0.0;
@@ -299,12 +326,12 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
         if (fabs_(vy) < u_cut) vy = 0.0;
         double vz = old(5) + az * dt;
         if (fabs_(vz) < u_cut) vz = 0.0;
-        out.xd[g] = vx;
-        out.yd[g] = vy;
-        out.zd[g] = vz;
-        out.x[g] = old(0) + vx * dt;
-        out.y[g] = old(1) + vy * dt;
-        out.z[g] = old(2) + vz * dt;
+        at32(out.xd, g) = vx;
+        at32(out.yd, g) = vy;
+        at32(out.zd, g) = vz;
+        at32(out.x, g) = old(0) + vx * dt;
+        at32(out.y, g) = old(1) + vy * dt;
+        at32(out.z, g) = old(2) + vz * dt;
     };
     // Ring fill: node plane K goes into its slot in two steps: fetch_plane
     // issues the global loads into registers (coalesced rows of pw+1 nodes;
@@ -330,12 +357,12 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
         for (int m = 0; m < 2; ++m) {
             if (go && pin[m]) {
                 const int g = K * n * n + pg[m];
-                rn[m][0] = in.x[g];
-                rn[m][1] = in.y[g];
-                rn[m][2] = in.z[g];
-                rn[m][3] = in.xd[g];
-                rn[m][4] = in.yd[g];
-                rn[m][5] = in.zd[g];
+                rn[m][0] = at32(in.x, g);
+                rn[m][1] = at32(in.y, g);
+                rn[m][2] = at32(in.z, g);
+                rn[m][3] = at32(in.xd, g);
+                rn[m][4] = at32(in.yd, g);
+                rn[m][5] = at32(in.zd, g);
             } else {
 #pragma unroll
                 for (int f = 0; f < 6; ++f) rn[m][f] = 0.0;
@@ -358,10 +385,10 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
     auto fetch_elem = [&](int k, bool go) {
         if (go && elemCol) {
             const int e = (k * s + ej) * s + ei;
-            ev[0] = p[e];
-            ev[1] = q[e];
-            ev[2] = volo[e];
-            ev[3] = v[e];
+            ev[0] = at32(p, e);
+            ev[1] = at32(q, e);
+            ev[2] = at32(volo, e);
+            ev[3] = at32(v, e);
         } else {
 #pragma unroll
             for (int f = 0; f < 4; ++f) ev[f] = 0.0;
@@ -386,38 +413,20 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
 
     const int kbeg = max(K0 - 1, 0);
     const int kend = min(K1, s); // element planes [kbeg, kend)
+    // nodal mass of this thread's node in the plane finished by the current
+    // step (fetched one step ahead, like the node planes)
+    auto fetch_mass = [&](int K) {
+        return ownsNode && K <= s ? at32(nodalMass, (K * n + J) * n + I) : 0.0;
+    };
     fetch_plane(kbeg, true);
     store_plane(kbeg);
     fetch_plane(kbeg + 1, kbeg < kend);
     fetch_elem(kbeg, kbeg < kend);
-    // nodal mass of this thread's node in the plane finished by the current
-    // step (fetched one step ahead, like the node planes)
-    auto fetch_mass = [&](int K) {
-        return ownsNode && K <= s ? nodalMass[(K * n + J) * n + I] : 0.0;
-    };
-    double mcur = fetch_mass(kbeg);
-    for (int k = kbeg; k < kend; ++k) {
-        // Plane k is in its slot; plane k+1 (fetched during the previous
-        // step) goes over plane k-1, whose last readers passed the previous
-        // step's second barrier (DBL: the previous element work and own_state
-        // read it; else: finish_node, before the third).  The previous step's
-        // last phase only read the corner staging area, which is written
-        // again after this barrier.
-        store_plane(k + 1);
-        __syncthreads();
-        double ox[6]; // (DBL) this thread's node of plane k, read while its slot is stable
-        if constexpr (DBL) {
-            if (ownsNode && k >= K0) own_state(k, ox);
-        }
-        // Wave priorities (s_setprio) for the workgroups sharing each SIMD:
-        // the element work issues ahead of the other workgroup's node phases
-        // (its LDS-bound corner sums and node updates fill the issue slots
-        // the FP64 chains leave), the hourglass part highest.  Measured at
-        // s=150: nodal 0.301 -> 0.261 ms (separate rounding), 0.243 -> 0.219
-        // ms (fused); no priority order tried was faster.
-        __builtin_amdgcn_s_setprio(2);
-        // ---- element (k, ej, ei): stress + hourglass corner forces ----------
-        double up[kStage]; // corners 4..7: (sx, sy, sz, hx, hy, hz) each, or their sums
+    double mcur = fetch_mass(kbeg), mnext = 0.0;
+    double up[kStage]; // corners 4..7: (sx, sy, sz, hx, hy, hz) each, or their sums
+    // ---- element (k, ej, ei): stress + hourglass corner forces ----------
+    // (lower corners to LDS or lo, upper corners to up; DBL: both staged)
+    auto element_work = [&](int k) {
         double lo[kStage]; // (LOREG) corners 0..3 likewise, staged with the upper ones
         // the running sum of lower-corner value c: in registers (LOREG) or in LDS
         auto lower = [&](int c) -> double & {
@@ -426,14 +435,24 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
             else
                 return lds[c][tid];
         };
+        // Wave priorities (s_setprio) for the waves sharing each SIMD: the
+        // element work issues ahead of the node phases of the other
+        // workgroup or team (its LDS-bound corner sums and node updates fill
+        // the issue slots the FP64 chains leave), the hourglass part
+        // highest.  Measured at s=150: nodal 0.301 -> 0.261 ms (separate
+        // rounding), 0.243 -> 0.219 ms (fused); no priority order tried was
+        // faster.
+        __builtin_amdgcn_s_setprio(2);
         if (elemCol) {
             // sigxx = sigyy = sigzz = -p - q (lulesh.cc:629-633) and the
             // hourglass determinant volo*v (lulesh.cc:1098)
             const int e = (k * s + ej) * s + ei;
-            const double sig = -ev[0] - ev[1];
+            // (times 1/16 for the scaled node normals: the products are the
+            // reference's bit for bit)
+            const double sig16 = (-ev[0] - ev[1]) * 0.0625;
             const double determ = ev[2] * ev[3];
-            const double sse = ss[e];
-            const double emass = elemMass[e];
+            const double sse = at32(ss, e);
+            const double emass = at32(elemMass, e);
             // the element's nodes in the patch node arrays (reference local
             // order, lulesh-init.cc:237-244): bottom plane k, top plane k+1
             const int q0 = jj * (pw + 1) + ii;
@@ -449,9 +468,8 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
             }
             {
                 double sc[3][8];
-                const double det = shape_derivatives<false>(x, y, z, nullptr);
-                if (det == 0) st->error = kVolumeError;
-                node_normals(sc, x, y, z);
+                if (jacobian_det_x64(x, y, z) == 0) st->error = kVolumeError;
+                node_normals_x16(sc, x, y, z);
                 // corners 0..3 go to LDS now (its last readers passed the
                 // previous barrier); corners 4..7 wait in registers
                 constexpr int W = COMBINE ? 3 : 6; // staged values per corner
@@ -460,10 +478,10 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
                         if constexpr (LOREG)
-                            lo[W * a + d] = -(sig * sc[d][a]);
+                            lo[W * a + d] = -(sig16 * sc[d][a]);
                         else
-                            lds[W * a + d][tid] = -(sig * sc[d][a]);
-                        up[W * a + d] = -(sig * sc[d][a + 4]);
+                            lds[W * a + d][tid] = -(sig16 * sc[d][a]);
+                        up[W * a + d] = -(sig16 * sc[d][a + 4]);
                     }
             }
             if (determ <= 0.0) st->error = kVolumeError;
@@ -547,14 +565,10 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
                 }
             }
         }
-        __builtin_amdgcn_sched_barrier(0); // keep the prefetch out of the element work
-        // the next step's plane, element scalars and nodal mass
-        fetch_plane(k + 2, k + 1 < kend);
-        fetch_elem(k + 1, k + 1 < kend);
-        const double mnext = fetch_mass(k + 1);
-        __syncthreads();
-        __builtin_amdgcn_s_setprio(0);
-        // ---- node plane k ends with corners 2,3,1,0 ----------------------------
+        __builtin_amdgcn_sched_barrier(0); // keep the prefetches out of the element work
+    };
+    // ---- node plane k ends with corners 2,3,1,0 ----------------------------
+    auto node_finish = [&](int k, const double (&ox)[6]) {
         if (ownsNode && k >= K0) {
             if (v00) add(2, tb);
             if (v01) add(3, tb + 1);
@@ -565,17 +579,11 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
             else
                 finish_node(k, mcur, [&](int f) { return pnv(slot_of(k), f, own); });
         }
-        if constexpr (!DBL) { // the upper corners over the lower ones, between barriers
-            __syncthreads();
-            if (elemCol) {
-#pragma unroll
-                for (int c = 0; c < kStage; ++c) lds[c][tid] = up[c];
-            }
-            __syncthreads();
-        }
-        // ---- node plane k+1 starts with corners 6,7,5,4 -------------------------
-        // the upper corners' rows (lds[kStage..] with DBL, else lds[0..])
-        // hold corners 4..7 at slots 0..3
+    };
+    // ---- node plane k+1 starts with corners 6,7,5,4 -------------------------
+    // the upper corners' rows (lds[kStage..] with DBL, else lds[0..]) hold
+    // corners 4..7 at slots 0..3
+    auto node_start = [&](int k) {
         constexpr int kUp = DBL ? kStage : 0;
         px = py = pz = 0.0;
         if (ownsNode && k + 1 < K1) {
@@ -584,6 +592,37 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
             if (v10) add(1, tb + pw, kUp);     // corner 5 of element (J, I-1)
             if (v11) add(0, tb + pw + 1, kUp); // corner 4 of element (J, I)
         }
+    };
+    for (int k = kbeg; k < kend; ++k) {
+        // Plane k is in its slot; plane k+1 (fetched during the previous
+        // step) goes over plane k-1, whose last readers passed the
+        // previous step's second barrier (DBL: the previous element work
+        // and own_state read it; else: finish_node, before the third).
+        // The previous step's last phase only read the corner staging
+        // area, which is written again after this barrier.
+        store_plane(k + 1);
+        __syncthreads();
+        double ox[6]; // (DBL) this thread's node of plane k, read while its slot is stable
+        if constexpr (DBL) {
+            if (ownsNode && k >= K0) own_state(k, ox);
+        }
+        element_work(k);
+        // the next step's plane, element scalars and nodal mass
+        fetch_plane(k + 2, k + 1 < kend);
+        fetch_elem(k + 1, k + 1 < kend);
+        mnext = fetch_mass(k + 1);
+        __syncthreads();
+        __builtin_amdgcn_s_setprio(0);
+        node_finish(k, ox);
+        if constexpr (!DBL) { // the upper corners over the lower ones, between barriers
+            __syncthreads();
+            if (elemCol) {
+#pragma unroll
+                for (int c = 0; c < kStage; ++c) lds[c][tid] = up[c];
+            }
+            __syncthreads();
+        }
+        node_start(k);
         mcur = mnext;
     }
     // The top node plane (K = s) has no element plane above it.
