@@ -606,9 +606,19 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
         if constexpr (DBL) {
             if (ownsNode && k >= K0) own_state(k, ox);
         }
+        // The mixed form fetches the next step's plane before the element
+        // work, so that the loads have the whole step to arrive (252 VGPRs;
+        // s=150: nodal 0.180 -> 0.158 ms); the fused FP64 form, whose element
+        // work is longer, gains nothing from it, and the separately rounded
+        // form would spill.
+        constexpr bool EARLY = MODE == MODE_MIXED;
+        if constexpr (EARLY) {
+            fetch_plane(k + 2, k + 1 < kend);
+            __builtin_amdgcn_sched_barrier(0);
+        }
         element_work(k);
         // the next step's plane, element scalars and nodal mass
-        fetch_plane(k + 2, k + 1 < kend);
+        if constexpr (!EARLY) fetch_plane(k + 2, k + 1 < kend);
         fetch_elem(k + 1, k + 1 < kend);
         mnext = fetch_mass(k + 1);
         __syncthreads();
