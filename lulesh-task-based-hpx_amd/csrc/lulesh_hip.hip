@@ -252,14 +252,13 @@ __device__ __forceinline__ void load8(const double *__restrict__ f, const int n[
 // nodes pair up along x (n1 = n0 + 1, n2 = n3 + 1, n5 = n4 + 1, n6 = n7 + 1).
 // A pair is 8-byte aligned; gfx950 reads it with one dwordx4 load.
 typedef double node_pair __attribute__((ext_vector_type(2)));
-// (n: indices from a base shared by the workgroup, as 32-bit offsets: at32)
 __device__ __forceinline__ void load8_pairs(const double *__restrict__ f, const int n[8],
                                             double o[8]) {
     node_pair p[4];
-    __builtin_memcpy(&p[0], &at32(f, n[0]), sizeof(node_pair));
-    __builtin_memcpy(&p[1], &at32(f, n[3]), sizeof(node_pair));
-    __builtin_memcpy(&p[2], &at32(f, n[4]), sizeof(node_pair));
-    __builtin_memcpy(&p[3], &at32(f, n[7]), sizeof(node_pair));
+    __builtin_memcpy(&p[0], f + n[0], sizeof(node_pair));
+    __builtin_memcpy(&p[1], f + n[3], sizeof(node_pair));
+    __builtin_memcpy(&p[2], f + n[4], sizeof(node_pair));
+    __builtin_memcpy(&p[3], f + n[7], sizeof(node_pair));
     o[0] = p[0].x; o[1] = p[0].y; o[3] = p[1].x; o[2] = p[1].y;
     o[4] = p[2].x; o[5] = p[2].y; o[7] = p[3].x; o[6] = p[3].y;
 }
@@ -453,30 +452,16 @@ __global__ __launch_bounds__(kBlock) void k_elems(
     const ProbDesc pd = pds[prob];
     const int li = (blk - pd.eBlock0) * blockDim.x + threadIdx.x;
     if (li >= pd.E || !st->active) return;
-    // Structured path: the arrays are addressed from this problem's first
-    // element and node (bases shared by the workgroup, so every access takes
-    // a 32-bit offset: at32); the indexed path (one problem) from the start.
-    const int i = STRUCT ? li : pd.elemOff + li;
-    const int eo = STRUCT ? pd.elemOff : 0;
+    const int i = pd.elemOff + li; // element index in the concatenated arrays
     const double deltaTime = st->deltatime;
-    double *const *nfa = nb.b[st->cur ^ st->fresh];
-    const double *nf[6];
-#pragma unroll
-    for (int f = 0; f < 6; ++f) nf[f] = nfa[f] + (STRUCT ? pd.nodeOff : 0);
-    ElemKin fe = f0;
-    fe.volo += eo;
-    fe.v += eo;
-    fe.vnew += eo;
-    fe.delv += eo;
-    fe.arealg += eo;
-    fe.vdov += eo;
+    double *const *nf = nb.b[st->cur ^ st->fresh];
     int n[8];
     if (STRUCT) {
         // element (k,j,i) -> its eight nodes, lulesh-init.cc:236-244
         const int s = pd.s, nn = s + 1;
         const int k = int((double(li) + 0.5) * pd.inv_ss), r = li - k * s * s;
         const int j = int((double(r) + 0.5) * pd.inv_s), ci = r - j * s;
-        const int b = (k * nn + j) * nn + ci;
+        const int b = pd.nodeOff + (k * nn + j) * nn + ci;
         n[0] = b; n[1] = b + 1; n[2] = b + nn + 1; n[3] = b + nn;
         n[4] = b + nn * nn; n[5] = b + nn * nn + 1; n[6] = b + nn * nn + nn + 1;
         n[7] = b + nn * nn + nn;
@@ -498,23 +483,23 @@ __global__ __launch_bounds__(kBlock) void k_elems(
     load8_node(nf[3], n, xd);
     load8_node(nf[4], n, yd);
     load8_node(nf[5], n, zd);
-    const double voloi = at32(fe.volo, i);
-    const double vold = at32(fe.v, i);
+    const double voloi = f0.volo[i];
+    const double vold = f0.v[i];
     const double volume = elem_volume<double>(x, y, z);
     const double rel = volume / voloi;
-    stream_put<NT>(&at32(fe.vnew, i), rel);
-    stream_put<NT>(&at32(fe.delv, i), rel - vold);
-    stream_put<NT>(&at32(fe.arealg, i), characteristic_length(x, y, z, volume));
+    stream_put<NT>(&f0.vnew[i], rel);
+    stream_put<NT>(&f0.delv[i], rel - vold);
+    stream_put<NT>(&f0.arealg[i], characteristic_length(x, y, z, volume));
     // Q gradients on the end-of-step geometry (lulesh.cc:1349)
     {
         QG g[6];
         element_q_gradients<QG>(x, y, z, xd, yd, zd, voloi * rel, g);
-        stream_put<NT>(&at32(dxxi + eo, i), g[0]);
-        stream_put<NT>(&at32(dxeta + eo, i), g[1]);
-        stream_put<NT>(&at32(dxzeta + eo, i), g[2]);
-        stream_put<NT>(&at32(dvxi + eo, i), g[3]);
-        stream_put<NT>(&at32(dveta + eo, i), g[4]);
-        stream_put<NT>(&at32(dvzeta + eo, i), g[5]);
+        stream_put<NT>(&dxxi[i], g[0]);
+        stream_put<NT>(&dxeta[i], g[1]);
+        stream_put<NT>(&dxzeta[i], g[2]);
+        stream_put<NT>(&dvxi[i], g[3]);
+        stream_put<NT>(&dveta[i], g[4]);
+        stream_put<NT>(&dvzeta[i], g[5]);
     }
     // Velocity divergence on the half-step geometry (lulesh.cc:1218-1239)
     const double dt2 = 0.5 * deltaTime;
@@ -526,13 +511,13 @@ __global__ __launch_bounds__(kBlock) void k_elems(
     }
     double B[3][8];
     const double detJ = shape_derivatives<true>(x, y, z, B);
-    stream_put<NT>(&at32(fe.vdov, i), velocity_divergence(xd, yd, zd, B, detJ));
+    stream_put<NT>(&f0.vdov[i], velocity_divergence(xd, yd, zd, B, detJ));
     // CheckAndUpdateVolumeForElems, lulesh.cc:1255-1270
     double vc = vold;
     if (eosvmin != 0.0 && vc < eosvmin) vc = eosvmin;
     if (eosvmax != 0.0 && vc > eosvmax) vc = eosvmax;
     if (vc < 0.0) st->error = kVolumeError;
-    stream_put<NT>(&at32(fe.v, i), (fabs_(rel - 1.0) < v_cut) ? 1.0 : rel);
+    stream_put<NT>(&f0.v[i], (fabs_(rel - 1.0) < v_cut) ? 1.0 : rel);
 }
 
 struct ElemFields {
