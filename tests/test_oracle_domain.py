@@ -75,6 +75,56 @@ def test_corner_lists_ascending_element_order():
     assert sorted(cl) == list(range(8 * s ** 3))
 
 
+def _gather(start, cl, a, b, order):
+    """Per-node sums of corner values a (stress) and b (hourglass) in a given
+    order, as Python float additions (IEEE doubles, left to right)."""
+    out = np.zeros(len(start) - 1)
+    for g in range(len(start) - 1):
+        corners = [int(c) for c in cl[start[g]:start[g + 1]]]
+        acc = 0.0
+        if order == "interleaved":        # lulesh.cc:849-857
+            for c in corners:
+                acc += float(a[c])
+                acc += float(b[c])
+        elif order == "stress_then_hourglass":
+            for c in corners:
+                acc += float(a[c])
+            for c in corners:
+                acc += float(b[c])
+        elif order == "descending":
+            for c in reversed(corners):
+                acc += float(a[c])
+                acc += float(b[c])
+        out[g] = acc
+    return out
+
+
+def test_node_force_gather_order():
+    """The oracle's node forces are the reference's gather
+    (combineVolumeForcesTaskFunc, lulesh.cc:838-862): per node, over its
+    corners in nodeElemCornerList order (ascending element index), stress
+    then hourglass force of each corner, one rounded add at a time.  Rebuilt
+    here from the oracle's own corner arrays (ora_calc_corner_forces, the
+    reference's fx_elem layout [8 elem + corner]) and held to the oracle's
+    fx/fy/fz bit for bit, on a state whose hourglass forces are non-zero;
+    the two plausible slips -- all stress before all hourglass forces, or
+    the corners in descending order -- change the bits, so the comparison
+    pins the order."""
+    s = 6
+    d = OracleDomain(s)
+    assert d.run(25) == 0
+    start, cl = d.i32("nodeElemStart"), d.i32("nodeElemCornerList")
+    rc, (sx, sy, sz, hx, hy, hz) = d.corner_forces()
+    assert rc == 0
+    assert np.max(np.abs(hx)) > 0 and np.max(np.abs(sx)) > 0
+    assert d.nodal() == 0
+    for f, a, b in (("fx", sx, hx), ("fy", sy, hy), ("fz", sz, hz)):
+        got = d.f64(f)
+        np.testing.assert_array_equal(got, _gather(start, cl, a, b, "interleaved"), err_msg=f)
+        for wrong in ("stress_then_hourglass", "descending"):
+            assert np.any(got != _gather(start, cl, a, b, wrong)), (f, wrong)
+
+
 def test_volumes_and_masses():
     s = 6
     d = OracleDomain(s)
