@@ -424,6 +424,24 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
     fetch_elem(kbeg, kbeg < kend);
     double mcur = fetch_mass(kbeg), mnext = 0.0;
     double up[kStage]; // corners 4..7: (sx, sy, sz, hx, hy, hz) each, or their sums
+    // CARRY (fused form): the coordinates of the element's bottom face are
+    // the previous step's top face, kept in registers (read from the ring
+    // once before the first step, after one more barrier): 12 fewer LDS
+    // reads per step at the start of the element work, where it waits on
+    // them (s=150: nodal 0.170 -> 0.168 ms).  The mixed form has no
+    // registers left for it; read inside the loop only for steps after the
+    // first, the compiler peeled the first step and the kernel was slower.
+    constexpr bool CARRY = MODE == MODE_FUSED;
+    double cxyz[3][4];
+    if constexpr (CARRY) {
+        __syncthreads(); // plane kbeg is in its ring slot
+        const int q0 = jj * (pw + 1) + ii;
+        const int nq[4] = {q0, q0 + 1, q0 + pw + 2, q0 + pw + 1};
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) cxyz[c][a] = elemCol ? pnv(slot_of(kbeg), c, nq[a]) : 0.0;
+    }
     // ---- element (k, ej, ei): stress + hourglass corner forces ----------
     // (lower corners to LDS or lo, upper corners to up; DBL: both staged)
     auto element_work = [&](int k) {
@@ -462,9 +480,23 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
             double x[8], y[8], z[8];
 #pragma unroll
             for (int a = 0; a < 8; ++a) {
-                x[a] = node(0, a);
-                y[a] = node(1, a);
-                z[a] = node(2, a);
+                if (CARRY && a < 4) { // the previous step's top face
+                    x[a] = cxyz[0][a];
+                    y[a] = cxyz[1][a];
+                    z[a] = cxyz[2][a];
+                } else {
+                    x[a] = node(0, a);
+                    y[a] = node(1, a);
+                    z[a] = node(2, a);
+                }
+            }
+            if constexpr (CARRY) {
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    cxyz[0][a] = x[a + 4];
+                    cxyz[1][a] = y[a + 4];
+                    cxyz[2][a] = z[a + 4];
+                }
             }
             {
                 double sc[3][8];
