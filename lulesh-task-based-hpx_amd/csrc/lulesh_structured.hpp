@@ -117,27 +117,17 @@ inline void elemmat_shape(int s, int &pw, int &ph) {
 // scalar registers and only the offset in a vector register (the saddr form
 // of global_load / global_store) instead of a 64-bit address per access.
 // The arrays of one problem stay below 4 GiB (at most 646^3 values of 8
-// bytes).
-#ifndef LULESH_SADDR
-#define LULESH_SADDR 1
-#endif
+// bytes).  In the nodal z-march: 1349 -> 1277 VALU instructions per element
+// step with the power-of-two-free stress integration, 0.174 -> 0.171 ms.
 template <typename T>
 __device__ __forceinline__ T &at32(T *base, int i) {
-#if LULESH_SADDR
     return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) +
                                   uint32_t(i) * uint32_t(sizeof(T)));
-#else
-    return base[i];
-#endif
 }
 template <typename T>
 __device__ __forceinline__ const T &at32(const T *base, int i) {
-#if LULESH_SADDR
     return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) +
                                         uint32_t(i) * uint32_t(sizeof(T)));
-#else
-    return base[i];
-#endif
 }
 
 struct NodeIn {
@@ -186,9 +176,12 @@ struct ProbDesc {
 //    hourglass force per corner before staging (one more rounding per
 //    corner, no change in what is summed), 12 staged values per element,
 //    and forms the hourglass force in two register-lean passes: 228 VGPRs
-//    without spills.  Two workgroups per CU.  At s=150 the nodal kernel takes
-//    0.185 ms against 0.211 ms staged as MODE_EXACT.
-//  MODE_MIXED (FP32 hourglass): staged as MODE_FUSED, 166 VGPRs.
+//    without spills.  Two workgroups per CU.  At s=150 the nodal kernel took
+//    0.185 ms against 0.211 ms staged as MODE_EXACT (now 0.168 ms, 226
+//    VGPRs with the bottom-face coordinates carried in registers, CARRY).
+//  MODE_MIXED (FP32 hourglass): staged as MODE_FUSED; with its lower
+//    corners' sums in registers (LOREG) and the next plane fetched before
+//    the element work (EARLY), 252 VGPRs, 0.149 ms at s=150.
 //  DBL (the fused and mixed forms): the upper corners get staging rows of
 //    their own (77 KB of LDS with the ring), so both corner sets are staged
 //    before one barrier, and each thread reads its node's old state from the
@@ -201,7 +194,7 @@ struct ProbDesc {
 // ms); the next plane copied into LDS by global_load_lds during the element
 // work (the copies' waits and spills cost more than the latency they hide;
 // fetching the next plane into registers before the element work instead of
-// after it is slower too).
+// after it gains nothing in the fused form, which the mixed form does).
 // SQUARE16: every problem of the launch has 16 x 16 patches (the shape is
 // then a compile-time constant; runtime shapes cost 2-6% at s=150).
 enum ZmarchMode { MODE_EXACT = 0, MODE_FUSED = 1, MODE_MIXED = 2 };
@@ -454,8 +447,8 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
                 return lds[c][tid];
         };
         // Wave priorities (s_setprio) for the waves sharing each SIMD: the
-        // element work issues ahead of the node phases of the other
-        // workgroup or team (its LDS-bound corner sums and node updates fill
+        // element work issues ahead of the node phase of the other
+        // workgroup (its LDS-bound corner sums and node updates fill
         // the issue slots the FP64 chains leave), the hourglass part
         // highest.  Measured at s=150: nodal 0.301 -> 0.261 ms (separate
         // rounding), 0.243 -> 0.219 ms (fused); no priority order tried was
