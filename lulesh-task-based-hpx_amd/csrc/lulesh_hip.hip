@@ -1353,9 +1353,9 @@ int flush_timers(lulesh_ctx *c) {
 
 enum { T_TIME = 0, T_NODAL, T_NODES, T_ELEMS, T_MATERIAL, T_MINIMA, T_ELEMMAT, T_COUNT };
 
-// Node planes per z-march workgroup when the tuning leaves it open: the
-// largest candidate that still gives the launch about eight workgroups per
-// CU (two resident per CU, four rounds); 4 for small problems.
+// Node planes per z-march workgroup when the tuning leaves it open (one
+// problem: the cost model below; a batch: the fewest plane groups that give
+// every CU a workgroup).
 int auto_zplanes(const lulesh_ctx *c) {
     if (c->nprob == 1) {
         // One problem: the count that minimises (rounds of workgroups) x
@@ -1396,8 +1396,17 @@ int auto_zplanes(const lulesh_ctx *c) {
         }
         return b;
     };
-    for (int z : {32, 24, 16, 12, 10, 8, 6, 5, 4})
-        if (z <= nmax && blocks(z) >= 2048) return z;
+    // A batch: the fewest plane groups along z (for its largest problem)
+    // that still give every CU a workgroup; the z-march's workgroups of a
+    // batch walk few planes, so each further group adds its recomputed
+    // plane and its prologue to little work.  Measured on the 64-problem
+    // batch (two streams, s = 21..80): one group per problem 8.08e9
+    // zone-cycles/s, two 7.97e9, the previous rule (>= 2048 workgroups, 12
+    // planes) 7.60e9.
+    for (int g = 1; g <= 40; ++g) {
+        const int z = (nmax + 2 * g - 2) / g; // the smallest z with at most g plane groups
+        if (z >= 2 && blocks(z) >= std::max(c->cus, 1)) return z;
+    }
     return std::min(4, nmax);
 }
 
@@ -3165,13 +3174,13 @@ int lulesh_hip_autotune(lulesh_ctx *c, int32_t cycles, lulesh_tuning *best,
     rc = trial(cur, &cur_ms);
     if (rc == 0) cur = cs[0]->tune;
     // coordinate descent over the launch parameters; the plane counts are
-    // the smallest ones giving 2..40 workgroups along z for the largest
+    // the smallest ones giving 1..40 workgroups along z for the largest
     // problem (each of them splits its node planes as evenly as a count can)
     int nmax = 1;
     for (lulesh_ctx *x : cs)
         for (const auto &p : x->probs) nmax = std::max(nmax, p.s + 1);
     std::vector<int> zcand;
-    for (int g = 40; g >= 2; --g) {
+    for (int g = 40; g >= 1; --g) {
         const int z = (nmax + 2 * g - 2) / g; // the smallest z with at most g plane groups
         if (z >= 2 && (zcand.empty() || zcand.back() != z)) zcand.push_back(z);
     }
