@@ -170,8 +170,9 @@ struct ProbDesc {
 //    corner's stress and hourglass force are staged separately and the node
 //    adds them one after the other, corner by corner, in the reference's
 //    interleaved order (lulesh.cc:849-857): bitwise the reference
-//    arithmetic.  24 staged values per element (77 KB of LDS); about 260
-//    VGPRs, held to 256 (a handful of spilled values): two workgroups per CU.
+//    arithmetic.  24 staged values per element (77 KB of LDS); 212 VGPRs
+//    with the two-pass hourglass and the carried bottom face: two
+//    workgroups per CU.
 //  MODE_FUSED (FP64, multiply-adds fused): the element adds its stress and
 //    hourglass force per corner before staging (one more rounding per
 //    corner, no change in what is summed), 12 staged values per element,
@@ -417,14 +418,15 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
     fetch_elem(kbeg, kbeg < kend);
     double mcur = fetch_mass(kbeg), mnext = 0.0;
     double up[kStage]; // corners 4..7: (sx, sy, sz, hx, hy, hz) each, or their sums
-    // CARRY (fused form): the coordinates of the element's bottom face are
-    // the previous step's top face, kept in registers (read from the ring
-    // once before the first step, after one more barrier): 12 fewer LDS
-    // reads per step at the start of the element work, where it waits on
-    // them (s=150: nodal 0.170 -> 0.168 ms).  The mixed form has no
+    // CARRY (fused and separately rounded forms): the coordinates of the
+    // element's bottom face are the previous step's top face, kept in
+    // registers (read from the ring once before the first step, after one
+    // more barrier): 12 fewer LDS reads per step at the start of the element
+    // work, where it waits on them (s=150: nodal 0.170 -> 0.168 ms fused,
+    // 0.221 -> 0.218 ms separately rounded).  The mixed form has no
     // registers left for it; read inside the loop only for steps after the
     // first, the compiler peeled the first step and the kernel was slower.
-    constexpr bool CARRY = MODE == MODE_FUSED;
+    constexpr bool CARRY = MODE != MODE_MIXED;
     double cxyz[3][4];
     if constexpr (CARRY) {
         __syncthreads(); // plane kbeg is in its ring slot
@@ -558,24 +560,24 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
                             up[3 * (a - 4) + c] += f;
                     });
             } else {
-                double xd[8], yd[8], zd[8];
-#pragma unroll
-                for (int a = 0; a < 8; ++a) {
-                    xd[a] = node(3, a);
-                    yd[a] = node(4, a);
-                    zd[a] = node(5, a);
-                }
-                double hg[3][8];
+                // FP64 hourglass force of each corner staged beside its
+                // stress, in the two-pass form: the same products summed in
+                // the same order as hourglass_force (bit for bit), with fewer
+                // values live (214 against 256 VGPRs and spills: s=150 nodal
+                // 0.233 -> 0.221 ms)
+                double dvdx[8], dvdy[8], dvdz[8], hmx[4], hmy[4], hmz[4];
+                volume_derivative<double>(dvdx, dvdy, dvdz, x, y, z);
+                hourglass_modes<double>(x, y, z, hmx, hmy, hmz);
                 const double coefficient = -hgcoef * 0.01 * sse * emass / cbrt_(determ);
-                hourglass_force<double>(x, y, z, xd, yd, zd, determ, coefficient, hg);
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        lds[6 * c + 3 + d][tid] = hg[d][c];
-                        up[6 * c + 3 + d] = hg[d][c + 4];
-                    }
-                }
+                fb_hourglass_two_pass<double>(
+                    dvdx, dvdy, dvdz, hmx, hmy, hmz, 1.0 / determ, coefficient,
+                    [&](int c, int a) { return node(3 + c, a); },
+                    [&](int c, int a, double f) {
+                        if (a < 4)
+                            lds[6 * a + 3 + c][tid] = f;
+                        else
+                            up[6 * (a - 4) + 3 + c] = f;
+                    });
             }
         } else {
 #pragma unroll
