@@ -95,8 +95,8 @@ def test_full_length_run_vs_oracle(pyl, name, form):
         return fields[f]
 
     if form == "mixed":
-        # FP32 hourglass and Q gradients: the north-star bound on the origin
-        # energy, and the symmetry of plane 0 at FP32 level
+        # FP32 hourglass and Q gradients: origin energy within 1e-6 relative
+        # of the FP64 run, and the symmetry of plane 0 at FP32 level
         assert ts.cycle == meta["cycle"]
         e0 = meta["report"]["e0"]
         rel = abs(rep[0] - e0) / abs(e0)

@@ -1,6 +1,7 @@
-"""The C++ binding shown in INTEGRATION.md (make_view: a lulesh_domain_view
-built from a Domain's accessors) compiles and builds the view the library
-expects.  The snippet is taken from INTEGRATION.md as written and compiled
+"""The C++ binding shown in the repository's integration guide (make_view:
+a lulesh_domain_view built from a Domain's accessors) compiles and builds
+the view the library expects.  The snippet is taken from the guide as
+written and compiled
 with g++ against this repository's host Domain, which keeps the reference
 Domain's accessor names (lulesh.h:248-448); the program then compares the
 view field by field with the view the library builds itself
