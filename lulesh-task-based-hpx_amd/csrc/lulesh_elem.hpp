@@ -92,6 +92,24 @@ __device__ __forceinline__ double sqrt_normal(double x) {
     return __builtin_fma(d, h, g);
 }
 
+// x^(-1/3) for a positive x, within about an ulp, without a division:
+// x = m 2^(3q) with m in [1/4, 2), an FP32 estimate exp2(-log2(m) / 3) of
+// m^(-1/3) (relative error below 1e-6), two Newton steps for the reciprocal
+// cube root, y += y (1 - m y^3) / 3 (the error e becomes -2 e^2: ~1e-12,
+// then the last step's rounding), and the result scaled by 2^-q.
+__device__ __forceinline__ double rcbrt_pos(double x) {
+    const int e = __builtin_amdgcn_frexp_exp(x); // x = f 2^e, f in [1/2, 1)
+    const int q = int(__builtin_rintf(float(e) * (1.0f / 3.0f)));
+    const double m = __builtin_ldexp(x, -3 * q);
+    double y = double(__builtin_amdgcn_exp2f(__builtin_amdgcn_logf(float(m)) * (-1.0f / 3.0f)));
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const double r = __builtin_fma(-(m * y), y * y, 1.0);
+        y = __builtin_fma(y * (1.0 / 3.0), r, y);
+    }
+    return __builtin_ldexp(y, -q);
+}
+
 // Square root of a face-area norm plus ptiny (>= 1e-36): sqrt_normal in FP64.
 __device__ __forceinline__ double sqrt_area(double x) { return sqrt_normal(x); }
 __device__ __forceinline__ float sqrt_area(float x) { return sqrt_(x); }
@@ -294,12 +312,43 @@ __device__ __forceinline__ void node_normals_x16(double pf[3][8], const double x
     face_normal_x16(pf, 4, 7, 6, 5, x, y, z);
 }
 
-// VoluDer, lulesh.cc:468-491
-template <typename T>
+// The node normals of the multiply-add form, from the face diagonals.  With
+// d1 = r[n2] - r[n0] and d2 = r[n3] - r[n1], the bisectors (without their
+// 0.5) are d1 + d2 and d1 - d2, and their cross product is 2 (d2 x d1); so
+// 8 times node_normals is the sum of d2 x d1 over each node's three faces:
+// the same quantity with other roundings, in 120 operations against 216.
+// The sums start at -0.0, which the first addition folds away exactly.
+__device__ __forceinline__ void face_normal_x8(double pf[3][8], int n0, int n1, int n2, int n3,
+                                               const double x[8], const double y[8],
+                                               const double z[8]) {
+    const double d1x = x[n2] - x[n0], d1y = y[n2] - y[n0], d1z = z[n2] - z[n0];
+    const double d2x = x[n3] - x[n1], d2y = y[n3] - y[n1], d2z = z[n3] - z[n1];
+    const double ax = d2y * d1z - d2z * d1y;
+    const double ay = d2z * d1x - d2x * d1z;
+    const double az = d2x * d1y - d2y * d1x;
+    pf[0][n0] += ax; pf[0][n1] += ax; pf[0][n2] += ax; pf[0][n3] += ax;
+    pf[1][n0] += ay; pf[1][n1] += ay; pf[1][n2] += ay; pf[1][n3] += ay;
+    pf[2][n0] += az; pf[2][n1] += az; pf[2][n2] += az; pf[2][n3] += az;
+}
+
+__device__ __forceinline__ void node_normals_x8(double pf[3][8], const double x[8],
+                                                const double y[8], const double z[8]) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) pf[0][i] = pf[1][i] = pf[2][i] = -0.0;
+    face_normal_x8(pf, 0, 1, 2, 3, x, y, z);
+    face_normal_x8(pf, 0, 4, 5, 1, x, y, z);
+    face_normal_x8(pf, 1, 5, 6, 2, x, y, z);
+    face_normal_x8(pf, 2, 6, 7, 3, x, y, z);
+    face_normal_x8(pf, 3, 7, 4, 0, x, y, z);
+    face_normal_x8(pf, 4, 7, 6, 5, x, y, z);
+}
+
+// VoluDer, lulesh.cc:468-491 (X12: without the final factor 1/12)
+template <typename T, bool X12 = false>
 __device__ __forceinline__ void volu_der(T x0, T x1, T x2, T x3, T x4, T x5, T y0, T y1, T y2,
                                          T y3, T y4, T y5, T z0, T z1, T z2, T z3, T z4, T z5,
                                          T &dvdx, T &dvdy, T &dvdz) {
-    const T twelfth = T(1.0) / T(12.0);
+    const T twelfth = X12 ? T(1.0) : T(1.0) / T(12.0);
     dvdx = (y1 + y2) * (z0 + z1) - (y0 + y1) * (z1 + z2) + (y0 + y4) * (z3 + z4) -
            (y3 + y4) * (z0 + z4) - (y2 + y5) * (z3 + z5) + (y3 + y5) * (z2 + z5);
     dvdy = -(x1 + x2) * (z0 + z1) + (x0 + x1) * (z1 + z2) - (x0 + x4) * (z3 + z4) +
@@ -311,12 +360,12 @@ __device__ __forceinline__ void volu_der(T x0, T x1, T x2, T x3, T x4, T x5, T y
     dvdz *= twelfth;
 }
 
-// CalcElemVolumeDerivative, lulesh.cc:495-523
-template <typename T>
+// CalcElemVolumeDerivative, lulesh.cc:495-523 (X12: the derivatives times 12)
+template <typename T, bool X12 = false>
 __device__ __forceinline__ void volume_derivative(T dvdx[8], T dvdy[8], T dvdz[8], const T x[8],
                                                   const T y[8], const T z[8]) {
 #define LULESH_VD(a, b, c, d, e, f, o)                                                       \
-    volu_der<T>(x[a], x[b], x[c], x[d], x[e], x[f], y[a], y[b], y[c], y[d], y[e], y[f], z[a], \
+    volu_der<T, X12>(x[a], x[b], x[c], x[d], x[e], x[f], y[a], y[b], y[c], y[d], y[e], y[f], z[a], \
                 z[b], z[c], z[d], z[e], z[f], dvdx[o], dvdy[o], dvdz[o])
     LULESH_VD(1, 2, 3, 4, 5, 7, 0);
     LULESH_VD(0, 1, 2, 7, 4, 6, 3);
@@ -406,6 +455,90 @@ __device__ __forceinline__ void fb_hourglass_two_pass(const T dvdx[8], const T d
         for (int c = 0; c < 3; ++c)
             sink(c, i, coefficient * (g[0] * hxx[c][0] + g[1] * hxx[c][1] + g[2] * hxx[c][2] +
                                       g[3] * hxx[c][3]));
+    }
+}
+
+// F(v)[m] = sum_i gamma[m][i] v[i] from pair sums and differences (18
+// additions instead of 28).
+template <typename T>
+__device__ __forceinline__ void gamma_forward(const T v[8], T h[4]) {
+    const T s01 = v[0] + v[1], s23 = v[2] + v[3], s45 = v[4] + v[5], s67 = v[6] + v[7];
+    const T d01 = v[0] - v[1], d23 = v[2] - v[3], d45 = v[4] - v[5], d67 = v[6] - v[7];
+    const T a = d01 + d23, b = d45 + d67;
+    h[0] = (s01 - s23) + (s67 - s45);
+    h[1] = (d01 - d23) + (d67 - d45);
+    h[2] = a + b;
+    h[3] = b - a;
+}
+
+// G(h)[i] = sum_m gamma[m][i] h[m], the transpose (12 additions instead of
+// 24; the signs of g[3] and g[5] fold into their users).
+template <typename T>
+__device__ __forceinline__ void gamma_transpose(const T h[4], T g[8]) {
+    const T p = h[0] + h[1], q = h[0] - h[1], r = h[2] + h[3], t = h[2] - h[3];
+    g[0] = p + t;
+    g[1] = q - t;
+    g[2] = t - p;
+    g[3] = -(q + t);
+    g[4] = r - p;
+    g[5] = -(q + r);
+    g[6] = p + r;
+    g[7] = q - r;
+}
+
+// CalcElemFBHourglassForce (lulesh.cc:527-562) with hourgam factored out:
+// hourgam[i][m] = gamma[m][i] - volinv (dvd_i . hm_m), so
+//   hxx[c][m] = F(v_c)[m] - volinv (hm_m . B_c),   B_c = sum_i dvd_i v_c[i]
+//   f[c][i]   = coefficient (G(hxx_c)[i] - volinv (dvd_i . H_c)),
+//               H_c = sum_m hm_m hxx[c][m]
+// with volinv folded into hm and the coefficient into hxx: 330 operations
+// against 472 for the row-by-row form.  The same sums in another order, so
+// not bitwise the reference's; the multiply-add form uses it.  hm are the
+// modes of the coordinates (F of x, y, z), vel(c, i) velocity component c
+// of node i, sink(c, i, force) takes each node force.
+template <typename T, typename Vel, typename Sink>
+__device__ __forceinline__ void fb_hourglass_factored(const T dvdx[8], const T dvdy[8],
+                                                      const T dvdz[8], const T hmx[4],
+                                                      const T hmy[4], const T hmz[4], T volinv,
+                                                      T coefficient, Vel vel, Sink sink) {
+    T hv[3][4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        hv[0][m] = volinv * hmx[m];
+        hv[1][m] = volinv * hmy[m];
+        hv[2][m] = volinv * hmz[m];
+    }
+    T hxx[3][4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        T v[8], f[4];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = vel(c, i);
+        gamma_forward<T>(v, f);
+        T bx = dvdx[0] * v[0], by = dvdy[0] * v[0], bz = dvdz[0] * v[0];
+#pragma unroll
+        for (int i = 1; i < 8; ++i) {
+            bx = bx + dvdx[i] * v[i];
+            by = by + dvdy[i] * v[i];
+            bz = bz + dvdz[i] * v[i];
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            hxx[c][m] = coefficient * (((f[m] - hv[0][m] * bx) - hv[1][m] * by) - hv[2][m] * bz);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        T hx = hv[0][0] * hxx[c][0], hy = hv[1][0] * hxx[c][0], hz = hv[2][0] * hxx[c][0];
+#pragma unroll
+        for (int m = 1; m < 4; ++m) {
+            hx = hx + hv[0][m] * hxx[c][m];
+            hy = hy + hv[1][m] * hxx[c][m];
+            hz = hz + hv[2][m] * hxx[c][m];
+        }
+        T g[8];
+        gamma_transpose<T>(hxx[c], g);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) sink(c, i, ((g[i] - dvdx[i] * hx) - dvdy[i] * hy) - dvdz[i] * hz);
     }
 }
 

@@ -178,8 +178,13 @@ struct ProbDesc {
 //    corner, no change in what is summed), 12 staged values per element,
 //    and forms the hourglass force in two register-lean passes: 228 VGPRs
 //    without spills.  Two workgroups per CU.  At s=150 the nodal kernel took
-//    0.185 ms against 0.211 ms staged as MODE_EXACT (now 0.168 ms, 226
-//    VGPRs with the bottom-face coordinates carried in registers, CARRY).
+//    0.185 ms against 0.211 ms staged as MODE_EXACT (0.168 ms with the
+//    bottom-face coordinates carried in registers, CARRY; now 0.155 ms, 232
+//    VGPRs, with fewer operations where the results need not be the
+//    reference's bit for bit: the hourglass force factored, the node
+//    normals from the face diagonals, a reciprocal cube root for the
+//    hourglass coefficient and volinv; 897 VALU instructions per element
+//    step against 982).
 //  MODE_MIXED (FP32 hourglass): staged as MODE_FUSED; with its lower
 //    corners' sums in registers (LOREG) and the next plane fetched before
 //    the element work (EARLY), 252 VGPRs, 0.149 ms at s=150.
@@ -460,9 +465,12 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
             // sigxx = sigyy = sigzz = -p - q (lulesh.cc:629-633) and the
             // hourglass determinant volo*v (lulesh.cc:1098)
             const int e = (k * s + ej) * s + ei;
-            // (times 1/16 for the scaled node normals: the products are the
-            // reference's bit for bit)
-            const double sig16 = (-ev[0] - ev[1]) * 0.0625;
+            // (times 1/16 for the scaled node normals, so that the products
+            // are the reference's bit for bit; 1/8 for the diagonal form of
+            // the multiply-add form: nodal 0.162 -> 0.158 ms at s=150; the
+            // mixed form ran 6% slower with it)
+            constexpr bool DIAG = MODE == MODE_FUSED;
+            const double sig16 = (-ev[0] - ev[1]) * (DIAG ? 0.125 : 0.0625);
             const double determ = ev[2] * ev[3];
             const double sse = at32(ss, e);
             const double emass = at32(elemMass, e);
@@ -496,7 +504,10 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
             {
                 double sc[3][8];
                 if (jacobian_det_x64(x, y, z) == 0) st->error = kVolumeError;
-                node_normals_x16(sc, x, y, z);
+                if constexpr (DIAG)
+                    node_normals_x8(sc, x, y, z);
+                else
+                    node_normals_x16(sc, x, y, z);
                 // corners 0..3 go to LDS now (its last readers passed the
                 // previous barrier); corners 4..7 wait in registers
                 constexpr int W = COMBINE ? 3 : 6; // staged values per corner
@@ -543,15 +554,23 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
                             up[3 * (a - 4) + c] += double(f);
                     });
             } else if constexpr (MODE == MODE_FUSED) {
-                // FP64 hourglass force added to each corner's stress in place,
-                // in two register-lean passes (velocities read from LDS when
-                // used; the sums are the one-pass form's)
+                // FP64 hourglass force added to each corner's stress in place
+                // (velocities read from LDS when used), with hourgam factored
+                // out and the factor 1/12 of the volume derivatives moved
+                // into volinv (982 -> 910 VALU instructions per element step,
+                // nodal 0.168 -> 0.161 ms at s=150)
                 double dvdx[8], dvdy[8], dvdz[8], hmx[4], hmy[4], hmz[4];
-                volume_derivative<double>(dvdx, dvdy, dvdz, x, y, z);
-                hourglass_modes<double>(x, y, z, hmx, hmy, hmz);
-                const double coefficient = -hgcoef * 0.01 * sse * emass / cbrt_(determ);
-                fb_hourglass_two_pass<double>(
-                    dvdx, dvdy, dvdz, hmx, hmy, hmz, 1.0 / determ, coefficient,
+                volume_derivative<double, true>(dvdx, dvdy, dvdz, x, y, z);
+                gamma_forward<double>(x, hmx);
+                gamma_forward<double>(y, hmy);
+                gamma_forward<double>(z, hmz);
+                // determ^(-1/3) without a division: the coefficient's divisor
+                // and, cubed, volinv (with the diagonal node normals: 897
+                // instructions, nodal 0.162 -> 0.155 ms)
+                const double rc = rcbrt_pos(determ);
+                const double coefficient = -hgcoef * 0.01 * sse * emass * rc;
+                fb_hourglass_factored<double>(
+                    dvdx, dvdy, dvdz, hmx, hmy, hmz, (rc * rc) * (rc * (1.0 / 12.0)), coefficient,
                     [&](int c, int a) { return node(3 + c, a); },
                     [&](int c, int a, double f) {
                         if (a < 4)
