@@ -110,6 +110,17 @@ __device__ __forceinline__ double rcbrt_pos(double x) {
     return __builtin_ldexp(y, -q);
 }
 
+// 1/x for a positive normal x, within an ulp, without the division's
+// range scaling and final correction: the hardware reciprocal estimate and
+// two Newton steps (each squares the relative error).
+__device__ __forceinline__ double recip_pos(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-x, y, 1.0);
+    return __builtin_fma(y, e, y);
+}
+
 // Square root of a face-area norm plus ptiny (>= 1e-36): sqrt_normal in FP64.
 __device__ __forceinline__ double sqrt_area(double x) { return sqrt_normal(x); }
 __device__ __forceinline__ float sqrt_area(float x) { return sqrt_(x); }
@@ -312,35 +323,94 @@ __device__ __forceinline__ void node_normals_x16(double pf[3][8], const double x
     face_normal_x16(pf, 4, 7, 6, 5, x, y, z);
 }
 
-// The node normals of the multiply-add form, from the face diagonals.  With
-// d1 = r[n2] - r[n0] and d2 = r[n3] - r[n1], the bisectors (without their
-// 0.5) are d1 + d2 and d1 - d2, and their cross product is 2 (d2 x d1); so
-// 8 times node_normals is the sum of d2 x d1 over each node's three faces:
-// the same quantity with other roundings, in 120 operations against 216.
-// The sums start at -0.0, which the first addition folds away exactly.
-__device__ __forceinline__ void face_normal_x8(double pf[3][8], int n0, int n1, int n2, int n3,
-                                               const double x[8], const double y[8],
+// The multiply-add form's transform of one coordinate r of the eight nodes
+// (local order, lulesh-init.cc:237-244: xi is + at nodes 1, 2, 5, 6, eta at
+// 2, 3, 6, 7, zeta at 4..7), by three butterfly stages (23 additions):
+// rxi, ret, rze = 8 times the Jacobian column entries of
+// CalcElemShapeFunctionDerivatives (lulesh.cc:316-328, the sums of r with
+// the signs of xi, eta, zeta) and hm = the hourglass modes hourmod
+// (lulesh.cc:974-988: gamma rows 0..3 are the signs of eta zeta, xi zeta,
+// xi eta and xi eta zeta).  Replaces 36 + 54 additions of the
+// reference-ordered Jacobian and modes.
+__device__ __forceinline__ void hex_transform(const double r[8], double &rxi, double &ret,
+                                              double &rze, double hm[4]) {
+    // along xi: node pairs 0-1, 3-2, 4-5, 7-6
+    const double sA = r[0] + r[1], dA = r[1] - r[0];
+    const double sB = r[3] + r[2], dB = r[2] - r[3];
+    const double sC = r[4] + r[5], dC = r[5] - r[4];
+    const double sD = r[7] + r[6], dD = r[6] - r[7];
+    // along eta: A, C at eta -, B, D at eta +
+    const double sAB = sA + sB, sCD = sC + sD; // 1
+    const double eAB = sB - sA, eCD = sD - sC; // eta
+    const double xAB = dA + dB, xCD = dC + dD; // xi
+    const double pAB = dB - dA, pCD = dD - dC; // xi eta
+    // along zeta: AB at zeta -, CD at zeta +
+    rxi = xAB + xCD;
+    ret = eAB + eCD;
+    rze = sCD - sAB;
+    hm[0] = eCD - eAB;
+    hm[1] = xCD - xAB;
+    hm[2] = pAB + pCD;
+    hm[3] = pCD - pAB;
+}
+
+// 64 times the determinant of the centre Jacobian (the value
+// CalcElemShapeFunctionDerivatives returns, lulesh.cc:330-373, up to
+// rounding) from the hex_transform columns; the stress path only tests it
+// against zero (lulesh.cc:819-822).
+__device__ __forceinline__ double jacobian_det_cols(double fjxxi, double fjxet, double fjxze,
+                                                    double fjyxi, double fjyet, double fjyze,
+                                                    double fjzxi, double fjzet, double fjzze) {
+    const double cjxet = fjzxi * fjyze - fjyxi * fjzze;
+    const double cjyet = fjxxi * fjzze - fjzxi * fjxze;
+    const double cjzet = fjyxi * fjxze - fjxxi * fjyze;
+    return fjxet * cjxet + fjyet * cjyet + fjzet * cjzet;
+}
+
+// The stress corner forces of the multiply-add form, -sig B with B the
+// node normals (CalcElemNodeNormals and SumElemStressesToNodeForces,
+// lulesh.cc:413-451 and 455-466), from the face diagonals.  With
+// d1 = r[n2] - r[n0] and d2 = r[n3] - r[n1], a face's bisectors (without
+// their 0.5) are d1 + d2 and d1 - d2, and their cross product is
+// 2 (d2 x d1); so 8 times a node's normal is the sum of d2 x d1 over its
+// three faces.  Each face vector is scaled by s = -sig / 8 and the three
+// are summed as (xi face + eta face) + zeta face, the four xi-eta pairs
+// shared by the nodes below and above: the reference's quantity with other
+// roundings, in 126 operations against 216 + 24.
+__device__ __forceinline__ void face_vector_x8(double a[3], int n0, int n1, int n2, int n3,
+                                               double s, const double x[8], const double y[8],
                                                const double z[8]) {
     const double d1x = x[n2] - x[n0], d1y = y[n2] - y[n0], d1z = z[n2] - z[n0];
     const double d2x = x[n3] - x[n1], d2y = y[n3] - y[n1], d2z = z[n3] - z[n1];
-    const double ax = d2y * d1z - d2z * d1y;
-    const double ay = d2z * d1x - d2x * d1z;
-    const double az = d2x * d1y - d2y * d1x;
-    pf[0][n0] += ax; pf[0][n1] += ax; pf[0][n2] += ax; pf[0][n3] += ax;
-    pf[1][n0] += ay; pf[1][n1] += ay; pf[1][n2] += ay; pf[1][n3] += ay;
-    pf[2][n0] += az; pf[2][n1] += az; pf[2][n2] += az; pf[2][n3] += az;
+    a[0] = s * (d2y * d1z - d2z * d1y);
+    a[1] = s * (d2z * d1x - d2x * d1z);
+    a[2] = s * (d2x * d1y - d2y * d1x);
 }
 
-__device__ __forceinline__ void node_normals_x8(double pf[3][8], const double x[8],
-                                                const double y[8], const double z[8]) {
+__device__ __forceinline__ void stress_forces_x8(double f[3][8], double s, const double x[8],
+                                                 const double y[8], const double z[8]) {
+    // the reference's faces (lulesh.cc:427-446): zeta-, eta-, xi+, eta+, xi-, zeta+
+    double zm[3], em[3], xp[3], ep[3], xm[3], zp[3];
+    face_vector_x8(zm, 0, 1, 2, 3, s, x, y, z);
+    face_vector_x8(em, 0, 4, 5, 1, s, x, y, z);
+    face_vector_x8(xp, 1, 5, 6, 2, s, x, y, z);
+    face_vector_x8(ep, 2, 6, 7, 3, s, x, y, z);
+    face_vector_x8(xm, 3, 7, 4, 0, s, x, y, z);
+    face_vector_x8(zp, 4, 7, 6, 5, s, x, y, z);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) pf[0][i] = pf[1][i] = pf[2][i] = -0.0;
-    face_normal_x8(pf, 0, 1, 2, 3, x, y, z);
-    face_normal_x8(pf, 0, 4, 5, 1, x, y, z);
-    face_normal_x8(pf, 1, 5, 6, 2, x, y, z);
-    face_normal_x8(pf, 2, 6, 7, 3, x, y, z);
-    face_normal_x8(pf, 3, 7, 4, 0, x, y, z);
-    face_normal_x8(pf, 4, 7, 6, 5, x, y, z);
+    for (int c = 0; c < 3; ++c) {
+        // nodes 0, 4: xi- eta-; 1, 5: xi+ eta-; 2, 6: xi+ eta+; 3, 7: xi- eta+
+        const double mm = xm[c] + em[c], pm = xp[c] + em[c];
+        const double pp = xp[c] + ep[c], mp = xm[c] + ep[c];
+        f[c][0] = mm + zm[c];
+        f[c][1] = pm + zm[c];
+        f[c][2] = pp + zm[c];
+        f[c][3] = mp + zm[c];
+        f[c][4] = mm + zp[c];
+        f[c][5] = pm + zp[c];
+        f[c][6] = pp + zp[c];
+        f[c][7] = mp + zp[c];
+    }
 }
 
 // VoluDer, lulesh.cc:468-491 (X12: without the final factor 1/12)

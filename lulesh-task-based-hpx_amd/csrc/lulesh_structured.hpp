@@ -179,12 +179,13 @@ struct ProbDesc {
 //    and forms the hourglass force in two register-lean passes: 228 VGPRs
 //    without spills.  Two workgroups per CU.  At s=150 the nodal kernel took
 //    0.185 ms against 0.211 ms staged as MODE_EXACT (0.168 ms with the
-//    bottom-face coordinates carried in registers, CARRY; now 0.155 ms, 232
+//    bottom-face coordinates carried in registers, CARRY; now 0.152 ms, 226
 //    VGPRs, with fewer operations where the results need not be the
-//    reference's bit for bit: the hourglass force factored, the node
-//    normals from the face diagonals, a reciprocal cube root for the
-//    hourglass coefficient and volinv; 897 VALU instructions per element
-//    step against 982).
+//    reference's bit for bit: the hourglass force factored, the stress
+//    forces from the face diagonals, one transform for the Jacobian
+//    columns and the hourglass modes, a reciprocal cube root for the
+//    hourglass coefficient and volinv, a Newton reciprocal for 1/mass;
+//    855 VALU instructions per element step against 982).
 //  MODE_MIXED (FP32 hourglass): staged as MODE_FUSED; with its lower
 //    corners' sums in registers (LOREG) and the next plane fetched before
 //    the element work (EARLY), 252 VGPRs, 0.149 ms at s=150.
@@ -296,8 +297,8 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
         const int g = (K * n + J) * n + I;
         double ax, ay, az;
         if constexpr (MODE == MODE_FUSED) {
-            // one division and three products (not bitwise the quotients)
-            const double rm = 1.0 / mass;
+            // a reciprocal and three products (not bitwise the quotients)
+            const double rm = recip_pos(mass);
             ax = px * rm;
             ay = py * rm;
             az = pz * rm;
@@ -466,11 +467,11 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
             // hourglass determinant volo*v (lulesh.cc:1098)
             const int e = (k * s + ej) * s + ei;
             // (times 1/16 for the scaled node normals, so that the products
-            // are the reference's bit for bit; 1/8 for the diagonal form of
-            // the multiply-add form: nodal 0.162 -> 0.158 ms at s=150; the
-            // mixed form ran 6% slower with it)
+            // are the reference's bit for bit; DIAG, the multiply-add form:
+            // the stress forces from the face diagonals, nodal 0.162 ->
+            // 0.158 ms at s=150; the mixed form ran 6% slower with them)
             constexpr bool DIAG = MODE == MODE_FUSED;
-            const double sig16 = (-ev[0] - ev[1]) * (DIAG ? 0.125 : 0.0625);
+            const double sig16 = (-ev[0] - ev[1]) * 0.0625;
             const double determ = ev[2] * ev[3];
             const double sse = at32(ss, e);
             const double emass = at32(elemMass, e);
@@ -501,11 +502,26 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
                     cxyz[2][a] = z[a + 4];
                 }
             }
+            // (DIAG) the coordinates' hourglass modes, from the transform
+            // that also gives the Jacobian columns (with the stress forces'
+            // shared face pairs and the reciprocal of the nodal mass: 897
+            // -> 855 instructions, nodal 0.155 -> 0.152 ms)
+            double hmc[3][4];
             {
                 double sc[3][8];
-                if (jacobian_det_x64(x, y, z) == 0) st->error = kVolumeError;
+                if constexpr (DIAG) {
+                    double jx[3], jy[3], jz[3];
+                    hex_transform(x, jx[0], jx[1], jx[2], hmc[0]);
+                    hex_transform(y, jy[0], jy[1], jy[2], hmc[1]);
+                    hex_transform(z, jz[0], jz[1], jz[2], hmc[2]);
+                    if (jacobian_det_cols(jx[0], jx[1], jx[2], jy[0], jy[1], jy[2], jz[0], jz[1],
+                                          jz[2]) == 0)
+                        st->error = kVolumeError;
+                } else {
+                    if (jacobian_det_x64(x, y, z) == 0) st->error = kVolumeError;
+                }
                 if constexpr (DIAG)
-                    node_normals_x8(sc, x, y, z);
+                    stress_forces_x8(sc, (ev[0] + ev[1]) * 0.125, x, y, z);
                 else
                     node_normals_x16(sc, x, y, z);
                 // corners 0..3 go to LDS now (its last readers passed the
@@ -515,11 +531,13 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
                 for (int a = 0; a < 4; ++a)
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
+                        const double lv = DIAG ? sc[d][a] : -(sig16 * sc[d][a]);
+                        const double uv = DIAG ? sc[d][a + 4] : -(sig16 * sc[d][a + 4]);
                         if constexpr (LOREG)
-                            lo[W * a + d] = -(sig16 * sc[d][a]);
+                            lo[W * a + d] = lv;
                         else
-                            lds[W * a + d][tid] = -(sig16 * sc[d][a]);
-                        up[W * a + d] = -(sig16 * sc[d][a + 4]);
+                            lds[W * a + d][tid] = lv;
+                        up[W * a + d] = uv;
                     }
             }
             if (determ <= 0.0) st->error = kVolumeError;
@@ -559,18 +577,16 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
                 // out and the factor 1/12 of the volume derivatives moved
                 // into volinv (982 -> 910 VALU instructions per element step,
                 // nodal 0.168 -> 0.161 ms at s=150)
-                double dvdx[8], dvdy[8], dvdz[8], hmx[4], hmy[4], hmz[4];
+                double dvdx[8], dvdy[8], dvdz[8];
                 volume_derivative<double, true>(dvdx, dvdy, dvdz, x, y, z);
-                gamma_forward<double>(x, hmx);
-                gamma_forward<double>(y, hmy);
-                gamma_forward<double>(z, hmz);
                 // determ^(-1/3) without a division: the coefficient's divisor
                 // and, cubed, volinv (with the diagonal node normals: 897
                 // instructions, nodal 0.162 -> 0.155 ms)
                 const double rc = rcbrt_pos(determ);
                 const double coefficient = -hgcoef * 0.01 * sse * emass * rc;
                 fb_hourglass_factored<double>(
-                    dvdx, dvdy, dvdz, hmx, hmy, hmz, (rc * rc) * (rc * (1.0 / 12.0)), coefficient,
+                    dvdx, dvdy, dvdz, hmc[0], hmc[1], hmc[2], (rc * rc) * (rc * (1.0 / 12.0)),
+                    coefficient,
                     [&](int c, int a) { return node(3 + c, a); },
                     [&](int c, int a, double f) {
                         if (a < 4)
