@@ -763,6 +763,79 @@ __device__ __forceinline__ void q_gradients(const T x[8], const T y[8], const T 
     out[4] = ax * dxv + ay * dyv + az * dzv;
 }
 
+// The multiply-add form's kinematics from three sums per nodal quantity
+// (local node order, lulesh-init.cc:237-244: xi is + at nodes 1, 2, 5, 6,
+// eta at 2, 3, 6, 7, zeta at 4..7): o = (sum xi_i r_i, sum eta_i r_i,
+// sum zeta_i r_i), in 17 additions.  They are 4 times the reference's
+// Q-gradient differences (dxi, dxj, dxk and dxv.. of lulesh.cc:1357-1415)
+// and 8 times its Jacobian columns (lulesh.cc:316-328); the scalings below
+// are powers of two, exact in every rounding.
+template <typename T>
+__device__ __forceinline__ void lin_sums(const T r[8], T o[3]) {
+    const T sA = r[0] + r[1], dA = r[1] - r[0];
+    const T sB = r[3] + r[2], dB = r[2] - r[3];
+    const T sC = r[4] + r[5], dC = r[5] - r[4];
+    const T sD = r[7] + r[6], dD = r[6] - r[7];
+    o[0] = (dA + dB) + (dC + dD);
+    o[1] = (sB - sA) + (sD - sC);
+    o[2] = (sC + sD) - (sA + sB);
+}
+
+// q_gradients (lulesh.cc:1349-1416) from the sums X[c][d] of coordinate c
+// and V[c][d] of velocity c along direction d: the face vectors are 16
+// times the reference's (ptiny scaled with them), their products with the
+// velocity sums 64 times.  out as in q_gradients.
+template <typename T>
+__device__ __forceinline__ void q_gradients_sums(const T X[3][3], const T V[3][3], T vol,
+                                                 T out[6]) {
+    const T ptiny = T(1.e-36);
+    const T norm = T(1.0 / 64.0) / (vol + ptiny);
+    const T vol16 = T(16.0) * vol;
+    // a x b for the directions a, b; the velocities along direction v
+    auto face = [&](int a, int b, int v, int oq, int ov) {
+        const T ax = X[1][a] * X[2][b] - X[2][a] * X[1][b];
+        const T ay = X[2][a] * X[0][b] - X[0][a] * X[2][b];
+        const T az = X[0][a] * X[1][b] - X[1][a] * X[0][b];
+        out[oq] = vol16 / sqrt_area(ax * ax + ay * ay + az * az + T(256.0) * ptiny);
+        out[ov] = (ax * V[0][v] + ay * V[1][v] + az * V[2][v]) * norm;
+    };
+    face(0, 1, 2, 2, 5); // zeta: xi x eta
+    face(1, 2, 0, 0, 3); // xi: eta x zeta
+    face(2, 0, 1, 1, 4); // eta: zeta x xi
+}
+
+// vdov, the trace of CalcElemVelocityGradient (lulesh.cc:733-753) on the
+// half-step geometry (lulesh.cc:1218-1239), from the sums: the half-step
+// Jacobian columns are J = X - dt2 V (8 times the reference's fj), with
+// cofactors C (64 times its cj) b[c][i] = sum_d C[c][d] sign_d(i) / 64 and
+// detJ = (J . C) / 64, so vdov = sum_cd C[c][d] V[c][d] / (J . C).
+__device__ __forceinline__ double vdov_sums(const double X[3][3], const double V[3][3],
+                                            double dt2) {
+    double J[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) J[c][d] = X[c][d] - dt2 * V[c][d];
+    double C[3][3];
+    C[0][0] = J[1][1] * J[2][2] - J[2][1] * J[1][2];
+    C[0][1] = J[2][0] * J[1][2] - J[1][0] * J[2][2];
+    C[0][2] = J[1][0] * J[2][1] - J[2][0] * J[1][1];
+    C[1][0] = J[2][1] * J[0][2] - J[0][1] * J[2][2];
+    C[1][1] = J[0][0] * J[2][2] - J[2][0] * J[0][2];
+    C[1][2] = J[2][0] * J[0][1] - J[0][0] * J[2][1];
+    C[2][0] = J[0][1] * J[1][2] - J[1][1] * J[0][2];
+    C[2][1] = J[1][0] * J[0][2] - J[0][0] * J[1][2];
+    C[2][2] = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+    const double det = J[0][1] * C[0][1] + J[1][1] * C[1][1] + J[2][1] * C[2][1];
+    double num = C[0][0] * V[0][0];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            if (c + d > 0) num = num + C[c][d] * V[c][d];
+    return num / det;
+}
+
 // One direction of the monotonic limiter, lulesh.cc:1457-1508: delvm/delvp
 // are the neighbour gradients already resolved through the BC switch.
 __device__ __forceinline__ double mono_phi(double self, double delvm, double delvp,

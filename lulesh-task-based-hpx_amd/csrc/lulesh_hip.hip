@@ -485,33 +485,74 @@ __global__ __launch_bounds__(kBlock) void k_elems(
     load8_node(nf[5], n, zd);
     const double voloi = f0.volo[i];
     const double vold = f0.v[i];
-    const double volume = elem_volume<double>(x, y, z);
-    const double rel = volume / voloi;
-    stream_put<NT>(&f0.vnew[i], rel);
-    stream_put<NT>(&f0.delv[i], rel - vold);
-    stream_put<NT>(&f0.arealg[i], characteristic_length(x, y, z, volume));
-    // Q gradients on the end-of-step geometry (lulesh.cc:1349)
-    {
-        QG g[6];
-        element_q_gradients<QG>(x, y, z, xd, yd, zd, voloi * rel, g);
+    const double dt2 = 0.5 * deltaTime;
+    auto put_gradients = [&](const QG g[6]) {
         stream_put<NT>(&dxxi[i], g[0]);
         stream_put<NT>(&dxeta[i], g[1]);
         stream_put<NT>(&dxzeta[i], g[2]);
         stream_put<NT>(&dvxi[i], g[3]);
         stream_put<NT>(&dveta[i], g[4]);
         stream_put<NT>(&dvzeta[i], g[5]);
-    }
-    // Velocity divergence on the half-step geometry (lulesh.cc:1218-1239)
-    const double dt2 = 0.5 * deltaTime;
+    };
+    double rel;
+    if constexpr (kFusedForm) {
+        // multiply-add and mixed forms: the Q gradients and the velocity
+        // divergence from three sums per coordinate and velocity component
+        // (the velocities reduced to theirs at once, the positions after
+        // the volume and the characteristic length): 744 -> 602 VALU
+        // instructions and 150 -> 126 VGPRs (FP64), s=150 kinematics
+        // 0.138 -> 0.135 ms
+        double V[3][3], X[3][3];
+        lin_sums<double>(xd, V[0]);
+        lin_sums<double>(yd, V[1]);
+        lin_sums<double>(zd, V[2]);
+        const double volume = elem_volume<double>(x, y, z);
+        rel = volume / voloi;
+        stream_put<NT>(&f0.vnew[i], rel);
+        stream_put<NT>(&f0.delv[i], rel - vold);
+        stream_put<NT>(&f0.arealg[i], characteristic_length(x, y, z, volume));
+        lin_sums<double>(x, X[0]);
+        lin_sums<double>(y, X[1]);
+        lin_sums<double>(z, X[2]);
+        QG g[6];
+        if constexpr (sizeof(QG) == 8) {
+            q_gradients_sums<double>(X, V, voloi * rel, g);
+        } else { // mixed form: the sums rounded to FP32 for the gradients
+            float Xf[3][3], Vf[3][3];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        x[k] -= dt2 * xd[k];
-        y[k] -= dt2 * yd[k];
-        z[k] -= dt2 * zd[k];
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    Xf[c][d] = float(X[c][d]);
+                    Vf[c][d] = float(V[c][d]);
+                }
+            q_gradients_sums<float>(Xf, Vf, float(voloi * rel), g);
+        }
+        put_gradients(g);
+        stream_put<NT>(&f0.vdov[i], vdov_sums(X, V, dt2));
+    } else {
+        const double volume = elem_volume<double>(x, y, z);
+        rel = volume / voloi;
+        stream_put<NT>(&f0.vnew[i], rel);
+        stream_put<NT>(&f0.delv[i], rel - vold);
+        stream_put<NT>(&f0.arealg[i], characteristic_length(x, y, z, volume));
+        // Q gradients on the end-of-step geometry (lulesh.cc:1349)
+        {
+            QG g[6];
+            element_q_gradients<QG>(x, y, z, xd, yd, zd, voloi * rel, g);
+            put_gradients(g);
+        }
+        // Velocity divergence on the half-step geometry (lulesh.cc:1218-1239)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            x[k] -= dt2 * xd[k];
+            y[k] -= dt2 * yd[k];
+            z[k] -= dt2 * zd[k];
+        }
+        double B[3][8];
+        const double detJ = shape_derivatives<true>(x, y, z, B);
+        stream_put<NT>(&f0.vdov[i], velocity_divergence(xd, yd, zd, B, detJ));
     }
-    double B[3][8];
-    const double detJ = shape_derivatives<true>(x, y, z, B);
-    stream_put<NT>(&f0.vdov[i], velocity_divergence(xd, yd, zd, B, detJ));
     // CheckAndUpdateVolumeForElems, lulesh.cc:1255-1270
     double vc = vold;
     if (eosvmin != 0.0 && vc < eosvmin) vc = eosvmin;
