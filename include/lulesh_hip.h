@@ -45,16 +45,20 @@ extern "C" {
 #define LULESH_PRECISION_FP64 0  /* every kernel in FP64 (the reference's Real_t, lulesh.h:27) */
 #define LULESH_PRECISION_MIXED 1 /* hourglass + monotonic-Q gradients in FP32, the rest FP64 */
 /* Option bits or-ed into the precision argument of lulesh_hip_create. */
-#define LULESH_OPT_NODE_FORCES 0x100 /* keep fx/fy/fz and xdd/ydd/zdd up to date in HBM, as
-                                        the reference's Domain does (lulesh.cc:858-860,
-                                        1115-1123).  The default of lulesh_hip_create (a
+#define LULESH_OPT_NODE_FORCES 0x100 /* keep the fields that are dead between cycles
+                                        up to date in HBM, as the reference's Domain
+                                        does: fx/fy/fz and xdd/ydd/zdd (lulesh.cc:858-860,
+                                        1115-1123) and the Q terms ql/qq (lulesh.cc:
+                                        1647-1648).  The default of lulesh_hip_create (a
                                         host Domain's view); batches and problems built in
                                         HBM keep them only with this bit. */
-#define LULESH_OPT_NO_NODE_FORCES 0x800 /* lulesh_hip_create only: do not write fx.. and
-                                        xdd.. (dead between cycles: nothing reads them
-                                        before the next cycle overwrites them).  Their
+#define LULESH_OPT_NO_NODE_FORCES 0x800 /* lulesh_hip_create only: do not write fx..,
+                                        xdd.. and ql/qq (dead between cycles: nothing
+                                        reads them before the next cycle overwrites them;
+                                        ql/qq are read back only by the EOS of the same
+                                        phase, which keeps them in registers).  Their
                                         downloads are then refused, and
-                                        lulesh_hip_download_view leaves those six host
+                                        lulesh_hip_download_view leaves those eight host
                                         arrays untouched. */
 #define LULESH_OPT_GENERIC 0x200     /* use the indexed kernels (nodelist + corner CSR)
                                         even when the mesh is the structured Sedov cube */
@@ -187,8 +191,8 @@ int lulesh_hip_constraints_noalloc(const double *ss, const double *vdov, const d
 int lulesh_hip_download(lulesh_ctx *ctx, int field_id, void *dst, size_t count);
 int lulesh_hip_upload(lulesh_ctx *ctx, int field_id, const void *src, size_t count);
 /* Download every state field into a view (NULL pointers are skipped), and
- * the time state.  fx/fy/fz and xdd/ydd/zdd are skipped (their host arrays
- * left as they are) when the context does not keep them
+ * the time state.  fx/fy/fz, xdd/ydd/zdd and ql/qq are skipped (their host
+ * arrays left as they are) when the context does not keep them
  * (LULESH_OPT_NO_NODE_FORCES, or a context built without
  * LULESH_OPT_NODE_FORCES by the batch or in-HBM set-up calls); a single
  * lulesh_hip_download of one of them then returns LULESH_INVALID_ARGUMENT. */

@@ -227,7 +227,7 @@ def test_benchmark_batch_of_64(pyl):
     problem run alone ends; three of them (the smallest mesh, which has 21
     regions; the largest; a 21-region s=52 one) match the oracle's
     full-length runs (tests/golden/oracle_batch64_p*.json) at 1e-9."""
-    from test_gpu_long import compare_with_fixture, load_fixture
+    from test_gpu_long import LIVE_ELEM, NODE, compare_with_fixture, load_fixture
 
     banner("benchmark batch: 64 problems, 1000 cycles")
     sizes, regions, factors = pyl.batch_params(0, 64)
@@ -253,5 +253,6 @@ def test_benchmark_batch_of_64(pyl):
         assert (meta["s"], meta["nr"]) == (int(sizes[k]), int(regions[k]))
         assert meta["factor"] == float(factors[k])
         print(f"[batch64] problem {k}: s={meta['s']} regions={meta['nr']}", flush=True)
-        compare_with_fixture(meta, arrays, lambda f: b.download(k, f), states[k], 1e-9)
+        compare_with_fixture(meta, arrays, lambda f: b.download(k, f), states[k], 1e-9,
+                             NODE + LIVE_ELEM)
     b.close()

@@ -29,7 +29,8 @@ def banner(msg):
 
 
 def run_mode(pyl, s, nr, b, c, prec, opts, mode, em_planes, cycles):
-    dev = pyl.Device.sedov(s, nr, b, c, precision=prec, opts=opts)
+    # (ql and qq, dead between cycles, are stored only with LULESH_OPT_NODE_FORCES)
+    dev = pyl.Device.sedov(s, nr, b, c, precision=prec, opts=opts | pyl.OPT_NODE_FORCES)
     t = dev.tuning()
     t.material_mode = mode
     t.em_planes = em_planes
@@ -86,7 +87,7 @@ def test_fused_pass_in_a_batch(pyl):
     sizes, regions, factors = pyl.batch_params(7, 9, 5, 40)
     res = []
     for mode in (0, 2):
-        bt = pyl.Batch.sedov(sizes, regions, factors)
+        bt = pyl.Batch.sedov(sizes, regions, factors, opts=pyl.OPT_NODE_FORCES)
         t = bt.tuning()
         t.material_mode = mode
         t.em_planes = 3 if mode == 2 else 0

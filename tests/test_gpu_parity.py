@@ -422,15 +422,16 @@ def test_elements_phase_reads_the_newest_node_state(pyl, path):
 
 
 def test_node_forces_kept_by_default(pyl):
-    """A context made from a host Domain keeps fx.. and xdd.. current as the
-    reference's Domain does; with LULESH_OPT_NO_NODE_FORCES their downloads
-    are refused and download_view leaves the host arrays alone."""
+    """A context made from a host Domain keeps fx.., xdd.. and ql, qq (the
+    fields dead between cycles) current as the reference's Domain does;
+    with LULESH_OPT_NO_NODE_FORCES their downloads are refused and
+    download_view leaves the host arrays alone, and the state is the same."""
     s = 6
     banner("node forces: default and opt-out")
     ora = OracleDomain(s)
     assert ora.run(12) == 0
     dom, dev, ts = pyl.run_device(s, 12)
-    for f in ("fx", "fy", "fz", "xdd", "ydd", "zdd"):
+    for f in ("fx", "fy", "fz", "xdd", "ydd", "zdd", "ql", "qq"):
         assert max_rel(dom.field(f), ora.f64(f)) < 1e-12, f
         assert np.any(dom.field(f) != 0), f
     dev.close()
@@ -438,10 +439,13 @@ def test_node_forces_kept_by_default(pyl):
     dev2 = pyl.Device(dom2, opts=pyl.OPT_NO_NODE_FORCES)
     ts2 = dom2.time_state()
     assert dev2.run(12, ts2) == 0
-    with pytest.raises(pyl.LuleshError):
-        dev2.download("fx")
-    dom2.field("fx")[:] = 7.0
+    for f in ("fx", "ql", "qq"):
+        with pytest.raises(pyl.LuleshError):
+            dev2.download(f)
+        dom2.field(f)[:] = 7.0
     dev2.download_view()
-    assert np.all(dom2.field("fx") == 7.0)
-    np.testing.assert_array_equal(dom2.field("e"), dom.field("e"))
+    for f in ("fx", "ql", "qq"):
+        assert np.all(dom2.field(f) == 7.0), f
+    for f in ("e", "p", "q", "ss", "v", "xd", "x"):
+        np.testing.assert_array_equal(dom2.field(f), dom.field(f), err_msg=f)
     dev2.close()
