@@ -48,30 +48,33 @@ def model_bytes_per_zone_cycle(s):
     return 816.0 + 204.0 * (s + 1) ** 3 / s ** 3
 
 
-def kernel_bytes(name, E, N, mixed=False, material_mode=0):
+def kernel_bytes(name, E, N, mixed=False, material_mode=0, keep_dead=False):
     """Algorithmic HBM bytes of one launch of each kernel over the mesh
     (structured path): every array element counted once per launch, node
-    arrays once per node."""
+    arrays once per node.  keep_dead: the context also stores the fields
+    that are dead between cycles (fx..zdd by the nodal kernel, ql qq by the
+    material kernels; LULESH_OPT_NODE_FORCES)."""
     gr = 4 if mixed else 8
+    dead_e = 16 if keep_dead else 0
     if name == "nodal":
         # z-march: node x..zd + nodalMass in, x..zd out; p q volo v ss elemMass in
-        return E * 48 + N * (48 + 8 + 48)
+        return E * 48 + N * (48 + 8 + 48 + (48 if keep_dead else 0))
     if name == "elements":
         # node x..zd in; volo v in; v vnew delv arealg vdov out; 6 gradients out
         return E * (16 + 40 + 6 * gr) + N * 48
     if name == "material":
         if material_mode == 0:
             # element order: repetition count (u16), elemBC, 6 gradients, vdov
-            # elemMass volo vnew e delv p q arealg in; qq ql p e q ss out
-            return E * (2 + 4 + 6 * gr + 72 + 48)
+            # elemMass volo vnew e delv p q arealg in; p e q ss out
+            return E * (2 + 4 + 6 * gr + 72 + 32 + dead_e)
         # region lists: list, elemBC, 6 gradients + 6 neighbour gradients
-        # (gathered), vdov elemMass volo vnew e delv p q arealg in; qq ql p e q ss out
-        return E * (4 + 4 + 12 * gr + 72 + 48)
+        # (gathered), vdov elemMass volo vnew e delv p q arealg in; p e q ss out
+        return E * (4 + 4 + 12 * gr + 72 + 32 + dead_e)
     if name == "elements_material":
         # the fused pass: node x..zd in; volo v in; v vnew delv arealg vdov out;
-        # repetition count (u16), elemBC, elemMass e p q in; qq ql p e q ss out
+        # repetition count (u16), elemBC, elemMass e p q in; p e q ss out
         # (the Q gradients stay on chip)
-        return E * (16 + 40 + 2 + 4 + 32 + 48) + N * 48
+        return E * (16 + 40 + 2 + 4 + 32 + 32 + dead_e) + N * 48
     if name == "time_increment":
         return 128
     return 0
