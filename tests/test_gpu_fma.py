@@ -125,16 +125,17 @@ def _max_rel(a, b):
 @pytest.mark.parametrize("path", ["structured", "generic"])
 @pytest.mark.parametrize("s", [8, 16, 31])
 def test_fma_phases_from_perturbed_state(pyl, s, path):
-    """One nodal and one kinematics phase of the contracted form against the
-    oracle from the same state: a developed state (30 oracle cycles) whose
-    velocities get seeded noise of 10% of their largest magnitude (every
-    element's hourglass modes excited) and whose coordinates get noise of 1%
-    of the mesh spacing (no element a parallelepiped).  The structured
-    kernels evaluate the hourglass force, the stress corner forces, the
-    Jacobian, the Q gradients and the velocity divergence in a rearranged
-    algebra; every result stays within 1e-12 of the oracle relative to the
-    field's largest magnitude, as the indexed kernels with only their
-    multiply-adds fused do."""
+    """One nodal, one kinematics and one material phase of the contracted
+    form against the oracle from the same state: a developed state (30
+    oracle cycles) whose velocities get seeded noise of 10% of their largest
+    magnitude (every element's hourglass modes excited) and whose
+    coordinates get noise of 1% of the mesh spacing (no element a
+    parallelepiped).  The contracted build evaluates the hourglass force,
+    the stress corner forces, the Jacobian, the Q gradients, the velocity
+    divergence, the EOS and the constraints in a rearranged algebra (the
+    structured nodal kernel) or with Newton reciprocals (the others); every
+    result stays within 1e-12 of the oracle relative to the field's largest
+    magnitude, and the time constraints within 1e-13."""
     banner(f"contracted phases from a perturbed state s={s} {path}")
     ora = OracleDomain(s)
     assert ora.run(30) == 0
@@ -167,6 +168,16 @@ def test_fma_phases_from_perturbed_state(pyl, s, path):
     for f in ["vnew", "delv", "arealg", "vdov", "v", "delv_xi", "delv_eta", "delv_zeta",
               "delx_xi", "delx_eta", "delx_zeta"]:
         worst[f] = _max_rel(dev.download(f), ora.f64(f))
+    for f in pyl.FIELDS:
+        dev.upload(f, ora.f64(f))
+    assert dev.phase(pyl.PHASE_MATERIAL, dt) == 0
+    assert ora.material() == 0
+    for f in ["e", "p", "q", "ql", "qq", "ss"]:
+        worst[f] = _max_rel(dev.download(f), ora.f64(f))
+    ora.constraints()
+    dtc, dth = dev.constraints_result()
+    assert abs(dtc - ora.scalar("dtcourant")) <= 1e-13 * ora.scalar("dtcourant")
+    assert abs(dth - ora.scalar("dthydro")) <= 1e-13 * ora.scalar("dthydro")
     print(f"[fma] worst {max(worst, key=worst.get)} {max(worst.values()):.2e}; "
           + " ".join(f"{k}={v:.1e}" for k, v in worst.items()), flush=True)
     for f, w in worst.items():
