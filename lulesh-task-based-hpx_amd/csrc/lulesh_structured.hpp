@@ -508,7 +508,7 @@ __attribute__((amdgpu_waves_per_eu(2))) void k_nodal_zmarch(
             // -> 855 instructions, nodal 0.155 -> 0.152 ms)
             double hmc[3][4];
             {
-                double sc[3][8];
+                double sc[3][8]; // node normals times 16, or (DIAG) the corner stress forces
                 if constexpr (DIAG) {
                     double jx[3], jy[3], jz[3];
                     hex_transform(x, jx[0], jx[1], jx[2], hmc[0]);
